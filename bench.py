@@ -1,0 +1,241 @@
+"""Throughput of the z-averaged 2D FD heat step on one MI355X.
+
+Workload (default, `--workload plate8k_f32`): an 8192 x 8192 plate, dx = dy
+= 5 um, dt = 1.2 us, k = 7 W/mK, rho = 4430, cp = 526, h = 50 um, T0 = 300 K
++ seeded uniform noise of +/-0.5 K, 20-track bidirectional raster with
+80 um hatch at P = 250 W, V = 1.2 m/s (the 50,000-step scan plan; the timed
+steps are its first K), mirror edges, FP32 field (theta = T - 300 K) with the
+fused per-step statistics (peak T + index, molten rows and count, energy
+sums) in FP64 across the grid.  One bench step = one time step of the whole
+plate.  `--workload plate8k_f64` runs the same plate on the FP64 parity path.
+
+Prints one JSON line: value = cell-updates/s over the timed steps, with the
+inputs already resident in HBM; `roofline` for the dominant kernel (step
+kernel duration from HIP events on its stream); `cpu_baseline` = the CPU
+oracle (FP64, one thread) timed on a bounded sample of the same plate.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def plate_setup(nx=8192, ny=8192):
+    import hedgehog_lpbf_amd as h
+    mat = h.Material(k=7.0, rho=4430.0, cp=526.0, Tm=1609.0, eps=0.40, eta=0.35)
+    pr = h.Process(P=250.0, V=1.2, r0=50e-6, h=50e-6, Tamb=300.0, Tsub=300.0)
+    dx = 5e-6
+    dom = h.Domain(Lx=(nx - 1) * dx, Ly=(ny - 1) * dx, nx=nx, ny=ny)
+    dt = 1.2e-6
+    n_plan = 50000
+    r = h.centred_raster(dom.Lx, dom.Ly, pr.V, dt, n_plan, 20, 80e-6)
+    plan = h.raster_plan(r["x0"], r["y0"], r["length"], r["hatch"], r["n_tracks"], r["V"],
+                         dt, n_plan)
+    k = h.make_const(mat, pr, dom, dt)
+    return mat, pr, dom, dt, k, plan
+
+
+def cpu_baseline(k, nx, ny, target_s=12.0):
+    """The CPU oracle (FP64 restatement, one thread) on the same plate."""
+    import numpy as np
+    from oracle import pyoracle as ora
+    T = 300.0 + ora.noise_field(0, 0.0, 0.5, (ny, nx))
+    kk = ora.StepConst.from_buffer_copy(bytes(k))
+    lib = ora.lib()
+    out = np.empty_like(T)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        lib.ora_step(ora._dp(T), ora._dp(out), C.byref(kk), 0)
+        T, out = out, T
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= target_s or steps >= 1000:
+            break
+    return {"value": nx * ny * steps / el, "unit": "cell-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle FP64 step (C, -O2, 1 thread), {nx}x{ny} plate, {steps} steps, "
+                      f"{el:.1f} s"}
+
+
+def traffic_for(workload):
+    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="plate8k_f32", choices=["plate8k_f32", "plate8k_f64"])
+    ap.add_argument("--depth", type=int, default=4)
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--n", type=int, default=8192, help="plate side (cells)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here "
+              "has one GPU; the multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import hedgehog_lpbf_amd as h
+
+    lib = h.lib()
+    mat, pr, dom, dt, k, plan = plate_setup(args.n, args.n)
+    nx, ny = dom.nx, dom.ny
+    cells = nx * ny
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    st = stream.cuda_stream
+    nsteps = args.warmup + args.steps
+    assert nsteps <= plan.shape[0]
+    pdev = torch.from_numpy(plan).to(dev)
+    stats = torch.empty(nsteps * 7 + 7, dtype=torch.float64, device=dev)
+    kaddr = C.addressof(k)
+    print(f"[bench] {args.workload} {nx}x{ny} warmup={args.warmup} steps={args.steps} "
+          f"depth={args.depth}", file=sys.stderr, flush=True)
+
+    if args.workload == "plate8k_f32":
+        depth, rows = args.depth, args.rows
+        ld = (nx + 3) // 4 * 4
+        a = torch.empty((ny, ld), dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        lib.lpbf_launch_noise_f32(a.data_ptr(), nx, ny, ld, 0, 0.0, 0.5, st)
+        work = torch.empty(lib.lpbf_f32_workspace_bytes(nx, ny, depth, rows) // 8 + 8,
+                           dtype=torch.float64, device=dev)
+        bytes_per_update = 8
+
+        def launch(s, d, bufs):
+            x, y = bufs
+            lib.lpbf_launch_step_f32(x.data_ptr(), y.data_ptr(), ld, kaddr, 300.0, 0, 300.0,
+                                     mat.Tm, pdev.data_ptr(), s, d, rows, None, work.data_ptr(),
+                                     st)
+
+        def finalize(s, d):
+            lib.lpbf_launch_f32_stats_finalize(work.data_ptr(), nx, ny, d, rows,
+                                               stats.data_ptr() + 56 * s, st)
+        kernel_name = "k_step_f32"
+    else:
+        depth = 1
+        a = torch.empty((ny, nx), dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        lib.lpbf_launch_noise_f64(a.data_ptr(), cells, 0, 300.0, 0.5, st)
+        nb = lib.lpbf_step_ex_nblocks(nx, ny)
+        work = torch.empty(nb * 7 + 7, dtype=torch.float64, device=dev)
+        bytes_per_update = 16
+        kk = h.StepConst.from_buffer_copy(bytes(k))
+
+        def launch(s, d, bufs):
+            x, y = bufs
+            kk.x_l, kk.y_l, kk.q_scale = plan[s]
+            lib.lpbf_launch_step_ex(x.data_ptr(), y.data_ptr(), C.addressof(kk), 0, 0, 300.0,
+                                    mat.Tm, work.data_ptr(), st)
+
+        def finalize(s, d):
+            lib.lpbf_launch_stats_finalize(work.data_ptr(), nb, nx, ny, stats.data_ptr() + 56 * s,
+                                           st)
+        kernel_name = "k_step_f64"
+
+    bufs = [a, b]
+
+    def run(s0, n, events=None):
+        s = s0
+        while s < s0 + n:
+            d = min(depth, s0 + n - s)
+            if events is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+            launch(s, d, bufs)
+            if events is not None:
+                e1.record(stream)
+                events.append((e0, e1, d))
+            finalize(s, d)
+            bufs.reverse()
+            s += d
+
+    run(0, args.warmup)
+    torch.cuda.synchronize(dev)
+    h.check()
+
+    events = []
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize(dev)
+    t_start.record(stream)
+    run(args.warmup, args.steps, events)
+    t_end.record(stream)
+    torch.cuda.synchronize(dev)
+    h.check()
+    total_ms = t_start.elapsed_time(t_end)
+    full = [(e0.elapsed_time(e1), d) for e0, e1, d in events if d == depth]
+    kern_ms = float(np.mean([t for t, _ in full])) if full else float("nan")
+
+    value = cells * args.steps / (total_ms * 1e-3)
+    alg_bytes = bytes_per_update * cells * depth
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    last = (h.StepStats * 1).from_buffer_copy(
+        stats.cpu().numpy().tobytes()[56 * (nsteps - 1): 56 * nsteps])[0]
+    print(f"[bench] total {total_ms:.2f} ms, step-kernel avg {kern_ms:.4f} ms per launch of "
+          f"{depth} steps, peak T {last.tmax:.1f} K, molten {last.molten}",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(k, nx, ny)
+
+    out = {
+        "metric": "cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.workload == "plate8k_f32" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{nx}x{ny} plate, dx 5 um, dt 1.2 us, 20-track raster P 250 W "
+                        f"V 1.2 m/s, T0 300 K +/-0.5 K noise, "
+                        f"{'FP32 theta field' if depth > 1 or args.workload == 'plate8k_f32' else 'FP64 field'}"
+                        f", fused per-step stats",
+            "nx": nx, "ny": ny, "temporal_depth": depth, "band_rows": args.rows,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kernel_name,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_for(args.workload),
+            "alg_bytes_per_launch": alg_bytes,
+            "kernel_ms_per_launch": kern_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

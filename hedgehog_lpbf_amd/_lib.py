@@ -1,0 +1,126 @@
+"""ctypes binding of libhedgehog_lpbf_amd.so (the gfx950 step library).
+
+The library is the product: every compute call below runs the HIP kernels.
+Loading fails loudly if the shared library has not been built; there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libhedgehog_lpbf_amd.so")
+
+LPBF_BC_MIRROR = 0
+LPBF_BC_DIRICHLET = 1
+LPBF_PHYS_BASELINE = 0
+LPBF_PHYS_EXTENDED = 1
+LPBF_ERR_INVALID_ARGUMENT = 1000
+
+
+class StepConst(C.Structure):
+    """include/lpbf_step_const.h (reference include/lpbf_step_const.hpp:21-32)."""
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("inv_dx2", C.c_double),
+        ("inv_dy2", C.c_double), ("inv_r0_sq", C.c_double),
+        ("alpha", C.c_double), ("K", C.c_double), ("rho", C.c_double),
+        ("cp", C.c_double), ("rho_cp", C.c_double), ("inv_h", C.c_double),
+        ("h", C.c_double), ("dt", C.c_double),
+        ("peak_S", C.c_double), ("eta", C.c_double), ("P", C.c_double),
+        ("hConv", C.c_double), ("Tamb", C.c_double), ("Tsub", C.c_double),
+        ("dSub", C.c_double), ("eps", C.c_double),
+        ("radiation", C.c_bool),
+        ("kmult", C.c_double),
+        ("x_l", C.c_double), ("y_l", C.c_double),
+        ("q_scale", C.c_double),
+    ]
+
+
+class Partial(C.Structure):
+    _fields_ = [("tmax", C.c_double), ("arg", C.c_longlong),
+                ("ymin", C.c_int), ("ymax", C.c_int)]
+
+
+class LaserTick(C.Structure):
+    _fields_ = [("x_l", C.c_double), ("y_l", C.c_double), ("q_scale", C.c_double)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("tmax", C.c_double), ("arg", C.c_longlong),
+                ("ymin", C.c_int), ("ymax", C.c_int),
+                ("molten", C.c_longlong), ("sum_dT", C.c_double),
+                ("sum_src", C.c_double), ("sum_loss", C.c_double)]
+
+
+assert C.sizeof(StepConst) == 216 and C.sizeof(Partial) == 24
+assert C.sizeof(LaserTick) == 24 and C.sizeof(StepStats) == 56
+
+_vp, _i, _d, _v = C.c_void_p, C.c_int, C.c_double, None
+_ll, _ull, _sz = C.c_longlong, C.c_ulonglong, C.c_size_t
+_DP = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); pointers to device memory and streams are
+# passed as integers (c_void_p).
+SIGNATURES = {
+    # reference seam (include/lpbf_kernels.h)
+    "lpbf_launch_step": (_v, [_vp, _vp, _vp, _i, _vp]),
+    "lpbf_launch_reduce": (_v, [_vp, _i, _i, _d, _vp, _i, _vp]),
+    "lpbf_launch_step_reduce": (_v, [_vp, _vp, _vp, _i, _d, _vp, _vp]),
+    "lpbf_launch_reduce_finalize": (_v, [_vp, _i, _i, _i, _vp, _vp]),
+    "lpbf_launch_extract_col": (_v, [_vp, _i, _i, _vp, _vp, _vp]),
+    # status
+    "lpbf_get_last_error": (_i, []),
+    "lpbf_error_string": (C.c_char_p, [_i]),
+    # FP64 additions
+    "lpbf_launch_step_tile": (_v, [_vp, _vp, _i, _i, _vp, _i, _vp]),
+    "lpbf_launch_step_ex": (_v, [_vp, _vp, _vp, _i, _i, _d, _d, _vp, _vp]),
+    "lpbf_step_ex_nblocks": (_i, [_i, _i]),
+    "lpbf_launch_stats_finalize": (_v, [_vp, _i, _i, _i, _vp, _vp]),
+    "lpbf_launch_noise_f64": (_v, [_vp, _ll, _ull, _d, _d, _vp]),
+    # FP32 path
+    "lpbf_f32_workspace_bytes": (_sz, [_i, _i, _i, _i]),
+    "lpbf_launch_step_f32": (_v, [_vp, _vp, _i, _vp, _d, _i, _d, _d, _vp, _i, _i, _i,
+                                  _vp, _vp, _vp]),
+    "lpbf_launch_f32_stats_finalize": (_v, [_vp, _i, _i, _i, _i, _vp, _vp]),
+    "lpbf_launch_noise_f32": (_v, [_vp, _i, _i, _i, _ull, _d, _d, _vp]),
+    "lpbf_launch_theta_to_T": (_v, [_vp, _i, _i, _i, _d, _vp, _vp]),
+    "lpbf_launch_T_to_theta": (_v, [_vp, _i, _i, _d, _vp, _i, _vp]),
+    # host helpers (include/lpbf_host.h)
+    "lpbf_make_const": (_v, [_DP, _DP, _i, _d, _d, _i, _i, _d, _vp]),
+    "lpbf_dt_cfl": (_d, [_DP, _d, _d, _i, _i]),
+    "lpbf_dt_cfl_ext": (_d, [_DP, _d, _d, _i, _i, _d]),
+    "lpbf_plan_single_track": (_v, [_d, _d, _d, _d, _i, _vp, _vp, _vp]),
+    "lpbf_plan_raster": (_v, [_d, _d, _d, _d, _i, _d, _d, _i, _vp]),
+}
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    """Load the library (once).  Raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() or "
+                "`make -C hedgehog_lpbf_amd/csrc`")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = h
+    return _LIB
+
+
+class LpbfError(RuntimeError):
+    pass
+
+
+def check() -> None:
+    """Raise if any lpbf_* call recorded an error since the last check."""
+    err = lib().lpbf_get_last_error()
+    if err:
+        raise LpbfError(f"lpbf error {err}: {lib().lpbf_error_string(err).decode()}")

@@ -1,0 +1,135 @@
+// SPDX-License-Identifier: 0BSD
+// Shared device helpers of the gfx950 step library: error recording, the
+// Partial / StepStats merge rules and wavefront (64-lane) reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "lpbf_step_const.h"
+
+namespace lpbf_amd {
+
+constexpr int kWave = 64;
+
+// Records the first error since the last lpbf_get_last_error() query.
+void record_error(int err);
+
+// After a launch: keep the runtime's sticky error visible to the caller
+// (reference convention, gpu_step_task.hpp:158) and record it here too.
+inline void check_launch() {
+    const hipError_t e = hipPeekAtLastError();
+    if (e != hipSuccess) record_error(static_cast<int>(e));
+}
+
+// Max value with the smallest row-major index on ties, min ymin, max ymax:
+// the merge of kernels.cu:229-233 and lpbf_cuda.cu:265-272.  It is exact,
+// associative and commutative, so any merge tree gives the serial result.
+struct PartialAcc {
+    double    tmax;
+    long long arg;
+    int       ymin, ymax;
+
+    __host__ __device__ static PartialAcc identity(int nx, int ny) {
+        return {-1.0, static_cast<long long>(nx) * ny, ny, -1};
+    }
+    __host__ __device__ void take_max(double v, long long a) {
+        if (v > tmax || (v == tmax && a < arg)) { tmax = v; arg = a; }
+    }
+    __host__ __device__ void merge(const PartialAcc& o) {
+        take_max(o.tmax, o.arg);
+        if (o.ymin < ymin) ymin = o.ymin;
+        if (o.ymax > ymax) ymax = o.ymax;
+    }
+};
+
+// Wavefront reduction of a PartialAcc; lane 0 ends with the merged value.
+__device__ inline PartialAcc wave_reduce(PartialAcc p) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        PartialAcc o;
+        o.tmax = __shfl_down(p.tmax, off);
+        o.arg = __shfl_down(p.arg, off);
+        o.ymin = __shfl_down(p.ymin, off);
+        o.ymax = __shfl_down(p.ymax, off);
+        p.merge(o);
+    }
+    return p;
+}
+
+// Fixed-shape tree sum over the wavefront (lane 0 holds the result); the
+// order is the same on every launch, so FP64 sums are reproducible.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off);
+    return v;
+}
+
+__device__ inline long long wave_sum(long long v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off);
+    return v;
+}
+
+__host__ __device__ inline StepStats stats_identity(int nx, int ny) {
+    StepStats s;
+    s.tmax = -1.0; s.arg = static_cast<long long>(nx) * ny;
+    s.ymin = ny; s.ymax = -1; s.molten = 0;
+    s.sum_dT = 0.0; s.sum_src = 0.0; s.sum_loss = 0.0;
+    return s;
+}
+
+// Merge parts[0..n) into *out with one 256-thread workgroup: thread t folds
+// t, t+256, ... in order, then a fixed tree, so FP64 sums are reproducible
+// and the Partial prefix equals the serial merge.
+__device__ inline void block_merge_stats(const StepStats* __restrict__ parts, int n, int nx,
+                                         int ny, StepStats* __restrict__ out) {
+    constexpr int kWaves = 256 / kWave;
+    PartialAcc p = PartialAcc::identity(nx, ny);
+    double sd = 0.0, ss = 0.0, sl = 0.0;
+    long long m = 0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const StepStats q = parts[i];
+        p.merge(PartialAcc{q.tmax, q.arg, q.ymin, q.ymax});
+        m += q.molten; sd += q.sum_dT; ss += q.sum_src; sl += q.sum_loss;
+    }
+    __shared__ PartialAcc s_pa[kWaves];
+    __shared__ double s_sum[3][kWaves];
+    __shared__ long long s_m[kWaves];
+    const PartialAcc w = wave_reduce(p);
+    const double wd = wave_sum(sd), ws = wave_sum(ss), wl = wave_sum(sl);
+    const long long wm = wave_sum(m);
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    if (lane == 0) {
+        s_pa[wave] = w; s_sum[0][wave] = wd; s_sum[1][wave] = ws; s_sum[2][wave] = wl;
+        s_m[wave] = wm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        PartialAcc b = s_pa[0];
+        StepStats st;
+        st.molten = s_m[0]; st.sum_dT = s_sum[0][0]; st.sum_src = s_sum[1][0];
+        st.sum_loss = s_sum[2][0];
+        for (int i = 1; i < kWaves; ++i) {
+            b.merge(s_pa[i]);
+            st.molten += s_m[i]; st.sum_dT += s_sum[0][i];
+            st.sum_src += s_sum[1][i]; st.sum_loss += s_sum[2][i];
+        }
+        st.tmax = b.tmax; st.arg = b.arg; st.ymin = b.ymin; st.ymax = b.ymax;
+        *out = st;
+    }
+}
+
+// SplitMix64 output function (Steele, Lea & Flood, OOPSLA 2014) applied to
+// the counter idx + (seed+1)*golden; returns u in [-0.5, 0.5) with 24 bits.
+__host__ __device__ inline double noise_u(unsigned long long seed,
+                                          unsigned long long idx) {
+    unsigned long long z = idx + (seed + 1ull) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return static_cast<double>(z >> 40) * (1.0 / 16777216.0) - 0.5;
+}
+
+}  // namespace lpbf_amd

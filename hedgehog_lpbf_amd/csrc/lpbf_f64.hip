@@ -1,0 +1,520 @@
+// SPDX-License-Identifier: 0BSD
+// FP64 step kernels for gfx950: the parity path of the library.
+//
+// Each cell update evaluates the reference expression in the reference's
+// operation order (include/step_task.hpp:93-129 / src/gpu/kernels.cu:106-147)
+// with FP contraction off, so every product and sum rounds as written and the
+// only difference from the CPU restatement is the device exp().
+//
+// Memory layout and schedule (MI355X-first):
+//   * a wavefront owns a 128-column x R-row block of the field; lane l holds
+//     columns c0+l and c0+64+l, so every row load is a contiguous 512-byte
+//     wavefront access;
+//   * the block is streamed top to bottom keeping the previous, current and
+//     next row in registers, so each input row is read from HBM once per
+//     block (plus one halo row above and below);
+//   * x neighbours come from cross-lane shuffles; lanes 0 and 63 load the
+//     one column to the left/right of the block;
+//   * four independent wavefronts form a 256-thread workgroup; reductions
+//     are wavefront shuffles followed by a 4-way merge in LDS.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include "lpbf_common.hpp"
+#include "lpbf_kernels.h"
+
+namespace lpbf_amd {
+namespace {
+
+constexpr double kSigmaSB = 5.670374419e-8;  // params.hpp:11
+constexpr int kBlockCols = 128;              // columns per wavefront
+constexpr int kThreads = 256;                // 4 wavefronts
+constexpr int kRowsPerWave = 32;
+
+// IN718 Mills tables (params.hpp:67-73) and their interpolation (:75-111).
+__constant__ double cT_PTS[9] = {298, 473, 673, 873, 1073, 1273, 1473, 1533, 1609};
+__constant__ double cK_PTS[9] = {8.9, 12.9, 15.8, 18.7, 21.9, 25.6, 28.6, 29.3, 29.6};
+__constant__ double cCP_PTS[9] = {435, 479, 515, 558, 580, 628, 670, 685, 720};
+
+__device__ inline double interp9(double T, const double* xs, const double* ys) {
+    if (T <= xs[0]) return ys[0];
+    if (T >= xs[8]) return ys[8];
+    int i = 1;
+    while (T > xs[i]) ++i;
+    const double f = (T - xs[i - 1]) / (xs[i] - xs[i - 1]);
+    return ys[i - 1] + f * (ys[i] - ys[i - 1]);
+}
+__device__ inline double k_of(double T) { return interp9(T, cT_PTS, cK_PTS); }
+__device__ inline double cp_of(double T) { return interp9(T, cT_PTS, cCP_PTS); }
+__device__ inline double k_enh(double T, double kmult) {
+    if (kmult <= 1.0) return 1.0;
+    double s;
+    if (T <= 1533.0) s = 0.0;
+    else if (T >= 1609.0) s = 1.0;
+    else s = (T - 1533.0) / (1609.0 - 1533.0);
+    return 1.0 + (kmult - 1.0) * s;
+}
+__device__ inline double k_eff(double T, double kmult) { return k_of(T) * k_enh(T, kmult); }
+__device__ inline double cp_eff(double T) {
+    double c = cp_of(T);
+    if (T >= 1533.0 && T <= 1609.0) c += 210.0e3 / (1609.0 - 1533.0);
+    return c;
+}
+
+// exp(x) is +0 for every x below this in both glibc and the device libm, and
+// then S = (q*peak*0)/(rho_cp*h) = +0, so the call can be skipped bit-exactly.
+constexpr double kExpZero = -746.0;
+
+// Baseline update (step_task.hpp:93-129).  xe: -1 at x = 0, +1 at x = nx-1.
+__device__ __forceinline__ double cell_base(double T, double Txm, double Txp,
+                                            double Tym, double Typ, int xe,
+                                            double xpos, double ypos,
+                                            const StepConst& k, double& src,
+                                            double& lossv) {
+    double lap_x;
+    if (xe < 0) lap_x = (Txp - T) * k.inv_dx2;
+    else if (xe > 0) lap_x = (Txm - T) * k.inv_dx2;
+    else lap_x = (Txp - 2.0 * T + Txm) * k.inv_dx2;
+    const double lap_y = (Typ - 2.0 * T + Tym) * k.inv_dy2;
+    const double r2 = (xpos - k.x_l) * (xpos - k.x_l) + (ypos - k.y_l) * (ypos - k.y_l);
+    const double arg = -2.0 * r2 * k.inv_r0_sq;
+    const double S = (arg < kExpZero)
+                         ? 0.0
+                         : k.q_scale * k.peak_S * exp(arg) / (k.rho_cp * k.h);
+    double q_top = k.hConv * (T - k.Tamb);
+    if (k.radiation) {
+        const double T4 = T * T * T * T;
+        const double Ta4 = k.Tamb * k.Tamb * k.Tamb * k.Tamb;
+        q_top += k.eps * kSigmaSB * (T4 - Ta4);
+    }
+    const double q_bot = (k.K / k.dSub) * (T - k.Tsub);
+    const double loss = (q_top + q_bot) * k.inv_h / k.rho_cp;
+    const double dTdt = k.alpha * (lap_x + lap_y) + S - loss;
+    src = k.dt * S;
+    lossv = k.dt * loss;
+    return T + k.dt * dTdt;
+}
+
+// Extended update (step_task.hpp:143-185); neighbours already ghost-substituted.
+__device__ __forceinline__ double cell_ext(double T, double Txm, double Txp,
+                                           double Tym, double Typ, double xpos,
+                                           double ypos, const StepConst& k,
+                                           double& src, double& lossv) {
+    const double km = k.kmult;
+    const double kCe = k_eff(T, km);
+    const double kxm = 0.5 * (kCe + k_eff(Txm, km));
+    const double kxp = 0.5 * (kCe + k_eff(Txp, km));
+    const double kym = 0.5 * (kCe + k_eff(Tym, km));
+    const double kyp = 0.5 * (kCe + k_eff(Typ, km));
+    const double div_k_grad = (kxp * (Txp - T) - kxm * (T - Txm)) * k.inv_dx2
+                            + (kyp * (Typ - T) - kym * (T - Tym)) * k.inv_dy2;
+    const double r2 = (xpos - k.x_l) * (xpos - k.x_l) + (ypos - k.y_l) * (ypos - k.y_l);
+    const double arg = -2.0 * r2 * k.inv_r0_sq;
+    const double q_in = (arg < kExpZero) ? 0.0 : k.q_scale * k.peak_S * exp(arg);
+    double q_top = k.hConv * (T - k.Tamb);
+    if (k.radiation) {
+        const double T4 = T * T * T * T;
+        const double Ta4 = k.Tamb * k.Tamb * k.Tamb * k.Tamb;
+        q_top += k.eps * kSigmaSB * (T4 - Ta4);
+    }
+    const double q_bot = (k_of(T) / k.dSub) * (T - k.Tsub);
+    const double rhs = div_k_grad + (q_in - q_top - q_bot) * k.inv_h;
+    const double rcp = k.rho * cp_eff(T);
+    src = k.dt * (q_in * k.inv_h) / rcp;
+    lossv = k.dt * ((q_top + q_bot) * k.inv_h) / rcp;
+    return T + k.dt * rhs / rcp;
+}
+
+enum { BC_MIRROR = LPBF_BC_MIRROR, BC_DIRICHLET = LPBF_BC_DIRICHLET, BC_TILE = 2 };
+enum { RED_NONE = 0, RED_PARTIAL = 1, RED_STATS = 2 };
+
+// The rows being updated.  For BC_TILE, in/out point at the first owned row
+// and rows -1 and ny (the halos) exist in both buffers.
+struct Region {
+    const double* in;
+    double* out;
+    int nx, ny;     // columns; rows to update
+    int y_off;      // global row of region row 0
+    int bc;
+    double T_bc;
+    double Tm;
+    int R;          // rows per wavefront block
+    int nwx;        // wavefront blocks across
+    int ntasks;     // wavefront blocks in total
+    int ny_glob;    // global row count (reduction identity)
+};
+
+template <int PHYS, int RED>
+__global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
+                                                       void* red_out, int red_fill) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int task = blockIdx.x * (kThreads / kWave) + wave;
+    const int nx = g.nx, ny = g.ny;
+
+    PartialAcc pa = PartialAcc::identity(nx, g.ny_glob);
+    long long molten = 0;
+    double s_dT = 0.0, s_src = 0.0, s_loss = 0.0;
+
+    if (task < g.ntasks) {
+        const int tx = task % g.nwx, ty = task / g.nwx;
+        const int c0 = tx * kBlockCols;
+        const int ya = ty * g.R;
+        const int yb = min(ya + g.R, ny);
+        const int x0 = c0 + lane, x1 = c0 + kWave + lane;
+        const bool a0 = x0 < nx, a1 = x1 < nx;
+        const int ecol = (lane == 0) ? c0 - 1 : c0 + kBlockCols;
+        const bool eok = (lane == 0 && c0 > 0) || (lane == kWave - 1 && c0 + kBlockCols < nx);
+        const bool tile = (g.bc == BC_TILE);
+
+        const double* rc = g.in + static_cast<long long>(ya) * nx;
+        double c0v = a0 ? rc[x0] : 0.0, c1v = a1 ? rc[x1] : 0.0;
+        double ce = eok ? rc[ecol] : 0.0;
+        double p0v = c0v, p1v = c1v;
+        if (ya > 0 || tile) {
+            const double* rp = g.in + static_cast<long long>(ya - 1) * nx;
+            p0v = a0 ? rp[x0] : 0.0;
+            p1v = a1 ? rp[x1] : 0.0;
+        }
+        if (tile) {  // halo rows pass through unchanged (step_task.hpp:78-84)
+            if (ya == 0) {
+                const long long o = -static_cast<long long>(nx);
+                if (a0) g.out[o + x0] = g.in[o + x0];
+                if (a1) g.out[o + x1] = g.in[o + x1];
+            }
+            if (yb == ny) {
+                const long long o = static_cast<long long>(ny) * nx;
+                if (a0) g.out[o + x0] = g.in[o + x0];
+                if (a1) g.out[o + x1] = g.in[o + x1];
+            }
+        }
+
+        for (int y = ya; y < yb; ++y) {
+            double n0v = c0v, n1v = c1v, ne = ce;
+            if (y + 1 < ny || tile) {
+                const double* rn = g.in + static_cast<long long>(y + 1) * nx;
+                n0v = a0 ? rn[x0] : 0.0;
+                n1v = a1 ? rn[x1] : 0.0;
+                ne = eok ? rn[ecol] : 0.0;
+            }
+            const double up0 = __shfl_up(c0v, 1), up1 = __shfl_up(c1v, 1);
+            const double dn0 = __shfl_down(c0v, 1), dn1 = __shfl_down(c1v, 1);
+            const double last0 = __shfl(c0v, kWave - 1), first1 = __shfl(c1v, 0);
+            const double xm[2] = {lane ? up0 : ce, lane ? up1 : last0};
+            const double xp[2] = {lane < kWave - 1 ? dn0 : first1,
+                                  lane < kWave - 1 ? dn1 : ce};
+            const double cv[2] = {c0v, c1v}, pv[2] = {p0v, p1v}, nv[2] = {n0v, n1v};
+            const int yg = g.y_off + y;
+            const double ypos = static_cast<double>(yg) * k.dy;
+            double* orow = g.out + static_cast<long long>(y) * nx;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int x = j ? x1 : x0;
+                if (!(j ? a1 : a0)) continue;
+                const double T = cv[j];
+                const double xpos = static_cast<double>(x) * k.dx;
+                double v, src = 0.0, lossv = 0.0;
+                bool upd = true;
+                if (g.bc == BC_DIRICHLET && (x == 0 || x == nx - 1 || y == 0 || y == ny - 1)) {
+                    v = g.T_bc;
+                    upd = false;
+                } else if (PHYS == LPBF_PHYS_BASELINE) {
+                    const int xe = (x == 0) ? -1 : (x == nx - 1 ? 1 : 0);
+                    v = cell_base(T, xm[j], xp[j], pv[j], nv[j], xe, xpos, ypos, k, src, lossv);
+                } else {
+                    const double Txm = (x == 0) ? T : xm[j];
+                    const double Txp = (x == nx - 1) ? T : xp[j];
+                    v = cell_ext(T, Txm, Txp, pv[j], nv[j], xpos, ypos, k, src, lossv);
+                }
+                orow[x] = v;
+                if (RED != RED_NONE) {
+                    pa.take_max(v, static_cast<long long>(yg) * nx + x);
+                    if (v > g.Tm) {
+                        if (yg < pa.ymin) pa.ymin = yg;
+                        if (yg > pa.ymax) pa.ymax = yg;
+                        ++molten;
+                    }
+                    if (RED == RED_STATS && upd) {
+                        s_dT += v - T;
+                        s_src += src;
+                        s_loss += lossv;
+                    }
+                }
+            }
+            p0v = c0v; p1v = c1v;
+            c0v = n0v; c1v = n1v; ce = ne;
+        }
+    }
+
+    if (RED == RED_NONE) return;
+    __shared__ PartialAcc s_pa[kThreads / kWave];
+    __shared__ double s_sum[3][kThreads / kWave];
+    __shared__ long long s_mol[kThreads / kWave];
+    const PartialAcc w = wave_reduce(pa);
+    double w_dT = 0.0, w_src = 0.0, w_loss = 0.0;
+    long long w_mol = 0;
+    if (RED == RED_STATS) {
+        w_dT = wave_sum(s_dT);
+        w_src = wave_sum(s_src);
+        w_loss = wave_sum(s_loss);
+        w_mol = wave_sum(molten);
+    }
+    if (lane == 0) {
+        s_pa[wave] = w;
+        s_sum[0][wave] = w_dT; s_sum[1][wave] = w_src; s_sum[2][wave] = w_loss;
+        s_mol[wave] = w_mol;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        PartialAcc b = s_pa[0];
+        for (int i = 1; i < kThreads / kWave; ++i) b.merge(s_pa[i]);
+        if (RED == RED_PARTIAL) {
+            Partial* out = static_cast<Partial*>(red_out);
+            out[blockIdx.x] = Partial{b.tmax, b.arg, b.ymin, b.ymax};
+        } else {
+            StepStats st;
+            st.tmax = b.tmax; st.arg = b.arg; st.ymin = b.ymin; st.ymax = b.ymax;
+            st.molten = s_mol[0]; st.sum_dT = s_sum[0][0];
+            st.sum_src = s_sum[1][0]; st.sum_loss = s_sum[2][0];
+            for (int i = 1; i < kThreads / kWave; ++i) {
+                st.molten += s_mol[i];
+                st.sum_dT += s_sum[0][i];
+                st.sum_src += s_sum[1][i];
+                st.sum_loss += s_sum[2][i];
+            }
+            static_cast<StepStats*>(red_out)[blockIdx.x] = st;
+        }
+    }
+    if (RED == RED_PARTIAL) {
+        // records beyond this grid (the reference sizes `out` for 16x16
+        // blocks) get the identity so a finalize over all of them is exact
+        Partial* out = static_cast<Partial*>(red_out);
+        const Partial id{-1.0, static_cast<long long>(nx) * g.ny_glob, g.ny_glob, -1};
+        for (int i = gridDim.x + blockIdx.x * kThreads + threadIdx.x; i < red_fill;
+             i += gridDim.x * kThreads)
+            out[i] = id;
+    }
+}
+
+// Reference reduce_kernel decomposition (kernels.cu:205-237): thread t of
+// block b visits b*256+t, stepping by 256*nblocks; per-thread strict '>'.
+__global__ __launch_bounds__(kThreads) void k_reduce_f64(const double* __restrict__ T,
+                                                         int nx, int ny, double Tm,
+                                                         Partial* __restrict__ out) {
+    const long long N = static_cast<long long>(nx) * ny;
+    PartialAcc p = PartialAcc::identity(nx, ny);
+    const long long stride = static_cast<long long>(kThreads) * gridDim.x;
+    for (long long i = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x; i < N;
+         i += stride) {
+        const double v = T[i];
+        if (v > p.tmax) { p.tmax = v; p.arg = i; }
+        if (v > Tm) {
+            const int yy = static_cast<int>(i / nx);
+            if (yy < p.ymin) p.ymin = yy;
+            if (yy > p.ymax) p.ymax = yy;
+        }
+    }
+    __shared__ PartialAcc s_pa[kThreads / kWave];
+    const PartialAcc w = wave_reduce(p);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_pa[threadIdx.x / kWave] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        PartialAcc b = s_pa[0];
+        for (int i = 1; i < kThreads / kWave; ++i) b.merge(s_pa[i]);
+        out[blockIdx.x] = Partial{b.tmax, b.arg, b.ymin, b.ymax};
+    }
+}
+
+// One workgroup merges all partials: thread t folds t, t+256, ... in order,
+// then a fixed tree.  The merge is exact, so this equals the serial loop.
+__global__ __launch_bounds__(kThreads) void k_finalize(const Partial* __restrict__ parts,
+                                                       int nblocks, int nx, int ny,
+                                                       Partial* __restrict__ out) {
+    PartialAcc p = PartialAcc::identity(nx, ny);
+    for (int i = threadIdx.x; i < nblocks; i += kThreads) {
+        const Partial q = parts[i];
+        p.merge(PartialAcc{q.tmax, q.arg, q.ymin, q.ymax});
+    }
+    __shared__ PartialAcc s_pa[kThreads / kWave];
+    const PartialAcc w = wave_reduce(p);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_pa[threadIdx.x / kWave] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        PartialAcc b = s_pa[0];
+        for (int i = 1; i < kThreads / kWave; ++i) b.merge(s_pa[i]);
+        out[0] = Partial{b.tmax, b.arg, b.ymin, b.ymax};
+    }
+}
+
+// Same for StepStats; the FP64 sums use a fixed order (deterministic).
+__global__ __launch_bounds__(kThreads) void k_stats_finalize(const StepStats* __restrict__ parts,
+                                                             int nblocks, int nx, int ny,
+                                                             StepStats* __restrict__ out) {
+    block_merge_stats(parts, nblocks, nx, ny, out);
+}
+
+__global__ void k_extract_col(const double* __restrict__ T, int nx, int ny,
+                              const Partial* __restrict__ glob, double* __restrict__ col) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= ny) return;
+    const int x_hot = static_cast<int>(glob->arg % nx);
+    col[y] = T[static_cast<long long>(y) * nx + x_hot];
+}
+
+__global__ void k_noise_f64(double* __restrict__ T, long long n, unsigned long long seed,
+                            double base, double amp) {
+    for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * blockDim.x)
+        T[i] = base + amp * 2.0 * noise_u(seed, static_cast<unsigned long long>(i));
+}
+
+Region make_region(const double* in, double* out, int nx, int ny, int y_off, int bc,
+                   double T_bc, double Tm, int ny_glob) {
+    Region g;
+    g.in = in; g.out = out; g.nx = nx; g.ny = ny; g.y_off = y_off; g.bc = bc;
+    g.T_bc = T_bc; g.Tm = Tm; g.R = kRowsPerWave;
+    g.nwx = (nx + kBlockCols - 1) / kBlockCols;
+    g.ntasks = g.nwx * ((ny + g.R - 1) / g.R);
+    g.ny_glob = ny_glob;
+    return g;
+}
+
+int grid_of(const Region& g) {
+    const int per = kThreads / kWave;
+    return (g.ntasks + per - 1) / per;
+}
+
+template <int RED>
+void launch_step(const Region& g, const StepConst& k, int phys, void* red, int fill,
+                 hipStream_t s) {
+    const dim3 grid(grid_of(g)), block(kThreads);
+    if (phys == LPBF_PHYS_EXTENDED)
+        hipLaunchKernelGGL((k_step_f64<LPBF_PHYS_EXTENDED, RED>), grid, block, 0, s, g, k, red, fill);
+    else
+        hipLaunchKernelGGL((k_step_f64<LPBF_PHYS_BASELINE, RED>), grid, block, 0, s, g, k, red, fill);
+    check_launch();
+}
+
+bool valid_grid(const StepConst* k) {
+    return k && k->nx >= 2 && k->ny >= 1 &&
+           static_cast<long long>(k->nx) * k->ny < (1LL << 40);
+}
+
+}  // namespace
+}  // namespace lpbf_amd
+
+using namespace lpbf_amd;
+
+extern "C" {
+
+void lpbf_launch_step(const double* Tin, double* Tout, const StepConst* k, int phys,
+                      hipStream_t stream) {
+    if (!valid_grid(k) || !Tin || !Tout || (phys != 0 && phys != 1)) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    const Region g = make_region(Tin, Tout, k->nx, k->ny, 0, BC_MIRROR, 0.0, 0.0, k->ny);
+    launch_step<RED_NONE>(g, *k, phys, nullptr, 0, stream);
+}
+
+void lpbf_launch_step_reduce(const double* Tin, double* Tout, const StepConst* k, int phys,
+                             double Tm, Partial* out, hipStream_t stream) {
+    if (!valid_grid(k) || !Tin || !Tout || !out || (phys != 0 && phys != 1)) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    const Region g = make_region(Tin, Tout, k->nx, k->ny, 0, BC_MIRROR, 0.0, Tm, k->ny);
+    const int fill = ((k->nx + 15) / 16) * ((k->ny + 15) / 16);
+    launch_step<RED_PARTIAL>(g, *k, phys, out, fill, stream);
+}
+
+void lpbf_launch_reduce(const double* T, int nx, int ny, double Tm, Partial* out,
+                        int nblocks, hipStream_t stream) {
+    if (!T || !out || nx < 1 || ny < 1 || nblocks < 1) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    hipLaunchKernelGGL(k_reduce_f64, dim3(nblocks), dim3(kThreads), 0, stream, T, nx, ny, Tm, out);
+    check_launch();
+}
+
+void lpbf_launch_reduce_finalize(const Partial* parts, int nblocks, int nx, int ny,
+                                 Partial* out, hipStream_t stream) {
+    if (!parts || !out || nblocks < 0) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kThreads), 0, stream, parts, nblocks, nx, ny, out);
+    check_launch();
+}
+
+void lpbf_launch_extract_col(const double* T, int nx, int ny, const Partial* glob,
+                             double* out_col, hipStream_t stream) {
+    if (!T || !glob || !out_col || nx < 1 || ny < 1) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    const int bs = 128;
+    hipLaunchKernelGGL(k_extract_col, dim3((ny + bs - 1) / bs), dim3(bs), 0, stream, T, nx, ny,
+                       glob, out_col);
+    check_launch();
+}
+
+void lpbf_launch_step_tile(const double* tile_in, double* tile_out, int owned_rows, int y0,
+                           const StepConst* k, int phys, hipStream_t stream) {
+    if (!k || k->nx < 2 || owned_rows < 1 || !tile_in || !tile_out ||
+        (phys != 0 && phys != 1)) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    const Region g = make_region(tile_in + k->nx, tile_out + k->nx, k->nx, owned_rows, y0,
+                                 BC_TILE, 0.0, 0.0, k->ny);
+    launch_step<RED_NONE>(g, *k, phys, nullptr, 0, stream);
+}
+
+int lpbf_step_ex_nblocks(int nx, int ny) {
+    if (nx < 1 || ny < 1) return 0;
+    const Region g = make_region(nullptr, nullptr, nx, ny, 0, BC_MIRROR, 0.0, 0.0, ny);
+    return grid_of(g);
+}
+
+void lpbf_launch_step_ex(const double* Tin, double* Tout, const StepConst* k, int phys, int bc,
+                         double T_bc, double Tm, StepStats* block_stats, hipStream_t stream) {
+    if (!valid_grid(k) || !Tin || !Tout || (phys != 0 && phys != 1) ||
+        (bc != LPBF_BC_MIRROR && bc != LPBF_BC_DIRICHLET)) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    const Region g = make_region(Tin, Tout, k->nx, k->ny, 0, bc, T_bc, Tm, k->ny);
+    if (block_stats)
+        launch_step<RED_STATS>(g, *k, phys, block_stats, 0, stream);
+    else
+        launch_step<RED_NONE>(g, *k, phys, nullptr, 0, stream);
+}
+
+void lpbf_launch_stats_finalize(const StepStats* parts, int nblocks, int nx, int ny,
+                                StepStats* out, hipStream_t stream) {
+    if (!parts || !out || nblocks < 0) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    hipLaunchKernelGGL(k_stats_finalize, dim3(1), dim3(kThreads), 0, stream, parts, nblocks, nx,
+                       ny, out);
+    check_launch();
+}
+
+void lpbf_launch_noise_f64(double* T, long long n, unsigned long long seed, double base,
+                           double amp, hipStream_t stream) {
+    if (!T || n < 0) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    if (n == 0) return;
+    const long long want = (n + kThreads - 1) / kThreads;
+    const int grid = static_cast<int>(want < 16384 ? want : 16384);
+    hipLaunchKernelGGL(k_noise_f64, dim3(grid), dim3(kThreads), 0, stream, T, n, seed, base, amp);
+    check_launch();
+}
+
+}  // extern "C"

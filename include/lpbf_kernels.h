@@ -1,0 +1,146 @@
+/* SPDX-License-Identifier: 0BSD
+ *
+ * C ABI of the MI355X (gfx950) step library libhedgehog_lpbf_amd.so.
+ *
+ * The first block keeps the names, argument order, argument meaning and
+ * calling convention of the reference launch seam (hedgehog_lpbf
+ * include/lpbf_kernels.hpp:19-55, defined at src/gpu/kernels.cu:280-319), with
+ * hipStream_t in place of cudaStream_t:
+ *   - the caller allocates and frees every device buffer;
+ *   - Tin and Tout never alias;
+ *   - *k is copied into the launch, so the caller may change it on return;
+ *   - every call is asynchronous on `stream`, returns nothing and never
+ *     synchronises; a launch failure or invalid argument is reported through
+ *     lpbf_get_last_error() (and the HIP runtime's hipGetLastError()).
+ *
+ * Fields are FP64 row-major nx*ny arrays (index y*nx + x) unless stated.
+ */
+#ifndef LPBF_KERNELS_H
+#define LPBF_KERNELS_H
+
+#include <hip/hip_runtime_api.h>
+#include "lpbf_step_const.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference seam ------------------------------------------------------ */
+
+/* One explicit-Euler step of the full grid, ghost = centre on all edges.
+ * phys 0 baseline, 1 extended (IN718 tables).
+ * Replaces lpbf_launch_step, include/lpbf_kernels.hpp:24-25. */
+void lpbf_launch_step(const double* Tin, double* Tout, const StepConst* k,
+                      int phys, hipStream_t stream);
+
+/* Grid-stride block partials of T: `nblocks` records into `out`, each the
+ * same record the reference reduce_kernel writes for that block.
+ * Replaces lpbf_launch_reduce, include/lpbf_kernels.hpp:30-31. */
+void lpbf_launch_reduce(const double* T, int nx, int ny, double Tm,
+                        Partial* out, int nblocks, hipStream_t stream);
+
+/* Step fused with the partial reduction of the new field.  Writes exactly
+ * ((nx+15)/16)*((ny+15)/16) records into `out` (records this kernel does not
+ * need hold the identity), so finalizing them with that count gives the
+ * global Partial of Tout.
+ * Replaces lpbf_launch_step_reduce, include/lpbf_kernels.hpp:37-39. */
+void lpbf_launch_step_reduce(const double* Tin, double* Tout,
+                             const StepConst* k, int phys, double Tm,
+                             Partial* out, hipStream_t stream);
+
+/* Merge `nblocks` partials into out[0] on the device (max value, smallest
+ * index on ties; min ymin; max ymax).
+ * Replaces lpbf_launch_reduce_finalize, include/lpbf_kernels.hpp:44-46. */
+void lpbf_launch_reduce_finalize(const Partial* parts, int nblocks, int nx,
+                                 int ny, Partial* out, hipStream_t stream);
+
+/* Copy column x_hot = glob->arg % nx of T into out_col[0..ny).
+ * Replaces lpbf_launch_extract_col, include/lpbf_kernels.hpp:51-53. */
+void lpbf_launch_extract_col(const double* T, int nx, int ny,
+                             const Partial* glob, double* out_col,
+                             hipStream_t stream);
+
+/* ---- status (new) ---------------------------------------------------------- */
+
+/* First error recorded by any lpbf_* call since the last query (a hipError_t
+ * value, or LPBF_ERR_INVALID_ARGUMENT), then cleared.  0 = success. */
+#define LPBF_ERR_INVALID_ARGUMENT 1000
+int         lpbf_get_last_error(void);
+const char* lpbf_error_string(int err);
+
+/* ---- FP64 additions (no reference counterpart) ------------------------------ */
+
+/* The reference StepTileTask::execute (include/step_task.hpp:54-206) on a
+ * device-resident row-strip tile: tile_in holds owned_rows+2 rows of k->nx
+ * values (halo row, owned rows, halo row; halos carry the mirror at the
+ * domain edges); the owned rows of tile_out receive the update, its halo rows
+ * a copy of tile_in's.  y0 is the global row of the first owned row. */
+void lpbf_launch_step_tile(const double* tile_in, double* tile_out,
+                           int owned_rows, int y0, const StepConst* k,
+                           int phys, hipStream_t stream);
+
+/* Generalised full-grid step: bc LPBF_BC_MIRROR (bitwise lpbf_launch_step) or
+ * LPBF_BC_DIRICHLET (edge cells set to T_bc, interior uses all four
+ * neighbours).  If block_stats != NULL it receives
+ * lpbf_step_ex_nblocks(nx, ny) block records of the new field, which
+ * lpbf_launch_stats_finalize merges into one StepStats. */
+void lpbf_launch_step_ex(const double* Tin, double* Tout, const StepConst* k,
+                         int phys, int bc, double T_bc, double Tm,
+                         StepStats* block_stats, hipStream_t stream);
+int  lpbf_step_ex_nblocks(int nx, int ny);
+
+/* Merge block StepStats in a fixed order (deterministic FP64 sums). */
+void lpbf_launch_stats_finalize(const StepStats* parts, int nblocks, int nx,
+                                int ny, StepStats* out, hipStream_t stream);
+
+/* Counter-based noise field: T[i] = base + 2*amp*u(seed, i), u in [-0.5,0.5)
+ * from SplitMix64 with 24 random bits; n values. */
+void lpbf_launch_noise_f64(double* T, long long n, unsigned long long seed,
+                           double base, double amp, hipStream_t stream);
+
+/* ---- FP32 path (no reference counterpart) ---------------------------------
+ *
+ * Field layout: theta = T - T_ref in FP32, row y at theta + y*ld, ld >= nx,
+ * ld % 4 == 0, base 16-byte aligned.  Columns [nx, ld) of every row are
+ * scratch: the step may overwrite them in theta_out.
+ *
+ * One launch advances `depth` (1..4) steps, theta_in -> theta_out, reading
+ * the laser for step step0+j from d_plan[step0+j] (device, FP64).  The grid
+ * is k->nx x k->ny with spacing and physics from *k (baseline physics; k's
+ * x_l, y_l, q_scale are ignored).  bc: LPBF_BC_MIRROR or LPBF_BC_DIRICHLET
+ * (edges held at T_bc).  `rows` is the band height of a wavefront tile
+ * (0 = built-in choice).  If d_stats != NULL it receives `depth` StepStats,
+ * one per step, of the fields after each step.  d_work must hold
+ * lpbf_f32_workspace_bytes(nx, ny, D, rows) bytes of device memory, which
+ * covers every launch of depth <= D with the same rows.
+ */
+size_t lpbf_f32_workspace_bytes(int nx, int ny, int depth, int rows);
+void   lpbf_launch_step_f32(const float* theta_in, float* theta_out, int ld,
+                            const StepConst* k, double T_ref, int bc,
+                            double T_bc, double Tm, const LaserTick* d_plan,
+                            int step0, int depth, int rows, StepStats* d_stats,
+                            void* d_work, hipStream_t stream);
+
+/* The per-step merge that lpbf_launch_step_f32 runs when d_stats != NULL,
+ * as its own call: after a step launch with d_stats == NULL and the same
+ * (nx, ny, depth, rows, d_work), writes the `depth` StepStats to d_stats. */
+void   lpbf_launch_f32_stats_finalize(const void* d_work, int nx, int ny,
+                                      int depth, int rows, StepStats* d_stats,
+                                      hipStream_t stream);
+
+/* theta = base + 2*amp*u(seed, y*nx+x) (same u as lpbf_launch_noise_f64). */
+void   lpbf_launch_noise_f32(float* theta, int nx, int ny, int ld,
+                             unsigned long long seed, double base, double amp,
+                             hipStream_t stream);
+
+/* Conversions between the FP64 row-major T and the pitched FP32 theta. */
+void   lpbf_launch_theta_to_T(const float* theta, int ld, int nx, int ny,
+                              double T_ref, double* T, hipStream_t stream);
+void   lpbf_launch_T_to_theta(const double* T, int nx, int ny, double T_ref,
+                              float* theta, int ld, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LPBF_KERNELS_H */

@@ -1,0 +1,86 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol the
+public headers declare, rejects invalid arguments without launching, and
+its host arithmetic (make_const, dt rules, laser plans) equals the oracle's
+bit for bit.  No GPU is touched."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import hedgehog_lpbf_amd as h
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for hdr in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        txt = open(hdr).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names |= set(re.findall(r"\b(lpbf_[A-Za-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = h.lib()
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(h._lib.SIGNATURES) == names
+
+
+def test_invalid_arguments_are_recorded():
+    lib = h.lib()
+    lib.lpbf_get_last_error()
+    k = h.StepConst()
+    k.nx, k.ny = 1, 4
+    lib.lpbf_launch_step(16, 32, C.addressof(k), 0, None)
+    assert lib.lpbf_get_last_error() == h._lib.LPBF_ERR_INVALID_ARGUMENT
+    assert lib.lpbf_get_last_error() == 0
+    k.nx = 64
+    lib.lpbf_launch_step(16, 32, C.addressof(k), 7, None)        # bad phys
+    assert lib.lpbf_get_last_error() == h._lib.LPBF_ERR_INVALID_ARGUMENT
+    lib.lpbf_launch_step_f32(16, 16, 64, C.addressof(k), 300.0, 0, 0.0, 1609.0, 16, 0, 1, 0,
+                             None, 16, None)                      # in == out
+    assert lib.lpbf_get_last_error() == h._lib.LPBF_ERR_INVALID_ARGUMENT
+    lib.lpbf_launch_step_f32(16, 48, 62, C.addressof(k), 300.0, 0, 0.0, 1609.0, 16, 0, 5, 0,
+                             None, 16, None)                      # depth 5, ld < nx
+    assert lib.lpbf_get_last_error() == h._lib.LPBF_ERR_INVALID_ARGUMENT
+    assert "invalid" in lib.lpbf_error_string(1000).decode()
+    with pytest.raises(h.LpbfError):
+        lib.lpbf_launch_reduce(16, 0, 4, 1609.0, 32, 1, None)
+        h.check()
+
+
+def test_make_const_and_dt_equal_oracle(ora):
+    for case in ("0", "1.1", "3.2"):
+        s = ora.defaults(case)
+        mat = h.Material()
+        pr = h.apply_case(case, h.Process())
+        dom = h.Domain()
+        assert h.dt_cfl(mat, dom) == ora.dt_cfl(s)
+        assert h.dt_cfl_ext(mat, dom, 2.0) == ora.dt_cfl_ext(s, 2.0)
+        dt = ora.dt_cfl(s)
+        a, b = h.make_const(mat, pr, dom, dt), ora.make_const(s, dt)
+        assert bytes(a) == bytes(b)
+
+
+def test_plans_equal_oracle(ora):
+    s = ora.defaults("0")
+    dt = ora.dt_cfl(s)
+    n = 1427
+    plan, x_csv, wrapped = h.single_track_plan(s.proc.V, dt, s.dom.Lx, s.dom.Ly, n)
+    assert np.array_equal(plan, ora.track_plan(s.proc.V, dt, s.dom.Lx, s.dom.Ly, n))
+    ticks = ora.build_plan(s.proc.V, dt, s.dom.Lx, s.dom.Ly, n, 200)
+    assert np.array_equal(x_csv, [t.x_laser_csv for t in ticks])
+    assert np.array_equal(wrapped, [bool(t.wrapped) for t in ticks])
+    r = h.centred_raster(40.96e-3, 40.96e-3, 1.2, 1.2e-6, 50000, 20, 80e-6)
+    got = h.raster_plan(r["x0"], r["y0"], r["length"], r["hatch"], r["n_tracks"], r["V"],
+                        1.2e-6, 50000)
+    want = ora.raster_plan(ora.OraRaster(**r), 1.2e-6, 50000)
+    assert np.array_equal(got, want)
+    assert got[:, 2].sum() > 0.9 * 50000 and got[-1, 2] in (0.0, 1.0)
