@@ -108,7 +108,6 @@ def main():
     assert nsteps <= plan.shape[0]
     pdev = torch.from_numpy(plan).to(dev)
     stats = torch.empty(nsteps * 7 + 7, dtype=torch.float64, device=dev)
-    kaddr = C.addressof(k)
     print(f"[bench] {args.workload} {nx}x{ny} warmup={args.warmup} steps={args.steps} "
           f"depth={args.depth}", file=sys.stderr, flush=True)
 
@@ -124,7 +123,7 @@ def main():
 
         def launch(s, d, bufs):
             x, y = bufs
-            lib.lpbf_launch_step_f32(x.data_ptr(), y.data_ptr(), ld, kaddr, 300.0, 0, 300.0,
+            lib.lpbf_launch_step_f32(x.data_ptr(), y.data_ptr(), ld, k, 300.0, 0, 300.0,
                                      mat.Tm, pdev.data_ptr(), s, d, rows, None, work.data_ptr(),
                                      st)
 
@@ -145,7 +144,7 @@ def main():
         def launch(s, d, bufs):
             x, y = bufs
             kk.x_l, kk.y_l, kk.q_scale = plan[s]
-            lib.lpbf_launch_step_ex(x.data_ptr(), y.data_ptr(), C.addressof(kk), 0, 0, 300.0,
+            lib.lpbf_launch_step_ex(x.data_ptr(), y.data_ptr(), kk, 0, 0, 300.0,
                                     mat.Tm, work.data_ptr(), st)
 
         def finalize(s, d):

@@ -98,7 +98,7 @@ def make_const(mat: Material, pr: Process, dom: Domain, dt: float) -> StepConst:
     k = StepConst()
     m, p = mat.arr(), pr.arr()
     lib().lpbf_make_const(_dp(m), _dp(p), int(pr.radiation), dom.Lx, dom.Ly, dom.nx, dom.ny,
-                          dt, C.addressof(k))
+                          dt, k)
     return k
 
 

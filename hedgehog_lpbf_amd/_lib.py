@@ -60,35 +60,37 @@ assert C.sizeof(LaserTick) == 24 and C.sizeof(StepStats) == 56
 _vp, _i, _d, _v = C.c_void_p, C.c_int, C.c_double, None
 _ll, _ull, _sz = C.c_longlong, C.c_ulonglong, C.c_size_t
 _DP = C.POINTER(C.c_double)
+_KP = C.POINTER(StepConst)   # host StepConst: pass the struct itself, which ctypes
+                             # keeps alive for the duration of the call
 
 # name -> (restype, argtypes); pointers to device memory and streams are
 # passed as integers (c_void_p).
 SIGNATURES = {
     # reference seam (include/lpbf_kernels.h)
-    "lpbf_launch_step": (_v, [_vp, _vp, _vp, _i, _vp]),
+    "lpbf_launch_step": (_v, [_vp, _vp, _KP, _i, _vp]),
     "lpbf_launch_reduce": (_v, [_vp, _i, _i, _d, _vp, _i, _vp]),
-    "lpbf_launch_step_reduce": (_v, [_vp, _vp, _vp, _i, _d, _vp, _vp]),
+    "lpbf_launch_step_reduce": (_v, [_vp, _vp, _KP, _i, _d, _vp, _vp]),
     "lpbf_launch_reduce_finalize": (_v, [_vp, _i, _i, _i, _vp, _vp]),
     "lpbf_launch_extract_col": (_v, [_vp, _i, _i, _vp, _vp, _vp]),
     # status
     "lpbf_get_last_error": (_i, []),
     "lpbf_error_string": (C.c_char_p, [_i]),
     # FP64 additions
-    "lpbf_launch_step_tile": (_v, [_vp, _vp, _i, _i, _vp, _i, _vp]),
-    "lpbf_launch_step_ex": (_v, [_vp, _vp, _vp, _i, _i, _d, _d, _vp, _vp]),
+    "lpbf_launch_step_tile": (_v, [_vp, _vp, _i, _i, _KP, _i, _vp]),
+    "lpbf_launch_step_ex": (_v, [_vp, _vp, _KP, _i, _i, _d, _d, _vp, _vp]),
     "lpbf_step_ex_nblocks": (_i, [_i, _i]),
     "lpbf_launch_stats_finalize": (_v, [_vp, _i, _i, _i, _vp, _vp]),
     "lpbf_launch_noise_f64": (_v, [_vp, _ll, _ull, _d, _d, _vp]),
     # FP32 path
     "lpbf_f32_workspace_bytes": (_sz, [_i, _i, _i, _i]),
-    "lpbf_launch_step_f32": (_v, [_vp, _vp, _i, _vp, _d, _i, _d, _d, _vp, _i, _i, _i,
+    "lpbf_launch_step_f32": (_v, [_vp, _vp, _i, _KP, _d, _i, _d, _d, _vp, _i, _i, _i,
                                   _vp, _vp, _vp]),
     "lpbf_launch_f32_stats_finalize": (_v, [_vp, _i, _i, _i, _i, _vp, _vp]),
     "lpbf_launch_noise_f32": (_v, [_vp, _i, _i, _i, _ull, _d, _d, _vp]),
     "lpbf_launch_theta_to_T": (_v, [_vp, _i, _i, _i, _d, _vp, _vp]),
     "lpbf_launch_T_to_theta": (_v, [_vp, _i, _i, _d, _vp, _i, _vp]),
     # host helpers (include/lpbf_host.h)
-    "lpbf_make_const": (_v, [_DP, _DP, _i, _d, _d, _i, _i, _d, _vp]),
+    "lpbf_make_const": (_v, [_DP, _DP, _i, _d, _d, _i, _i, _d, _KP]),
     "lpbf_dt_cfl": (_d, [_DP, _d, _d, _i, _i]),
     "lpbf_dt_cfl_ext": (_d, [_DP, _d, _d, _i, _i, _d]),
     "lpbf_plan_single_track": (_v, [_d, _d, _d, _d, _i, _vp, _vp, _vp]),

@@ -367,6 +367,13 @@ void launch_k(const F32Args& a, const LaserTick* plan, int step0, hipStream_t s)
                            plan, step0);
 }
 
+// The FP32 kernels index cells with 32-bit ints and rely on nx, ny for every
+// address; shapes outside this range are rejected.
+bool f32_shape_ok(long long nx, long long ny) {
+    return nx >= 2 && ny >= 1 && nx <= (1LL << 28) && ny <= (1LL << 28) &&
+           nx * ny < (1LL << 31) - 1;
+}
+
 F32Args make_args(const float* in, float* out, int ld, const StepConst& k, double T_ref, int bc,
                   double T_bc, double Tm, int depth, int rows, void* work) {
     F32Args a{};
@@ -409,7 +416,7 @@ using namespace lpbf_amd;
 extern "C" {
 
 size_t lpbf_f32_workspace_bytes(int nx, int ny, int depth, int rows) {
-    if (nx < 2 || ny < 1 || depth < 1 || depth > kMaxDepth || rows < 0) return 0;
+    if (!f32_shape_ok(nx, ny) || depth < 1 || depth > kMaxDepth || rows < 0) return 0;
     // enough for every depth up to `depth` (a run ends with shorter launches)
     size_t need = 0;
     for (int d = 1; d <= depth; ++d) {
@@ -424,12 +431,11 @@ void lpbf_launch_step_f32(const float* theta_in, float* theta_out, int ld, const
                           double T_ref, int bc, double T_bc, double Tm, const LaserTick* d_plan,
                           int step0, int depth, int rows, StepStats* d_stats, void* d_work,
                           hipStream_t stream) {
-    if (!k || k->nx < 2 || k->ny < 1 || !theta_in || !theta_out || theta_in == theta_out ||
+    if (!k || !f32_shape_ok(k->nx, k->ny) || !theta_in || !theta_out || theta_in == theta_out ||
         ld < k->nx || (ld & 3) || (reinterpret_cast<uintptr_t>(theta_in) & 15) ||
         (reinterpret_cast<uintptr_t>(theta_out) & 15) || !d_plan || step0 < 0 || depth < 1 ||
         depth > kMaxDepth || rows < 0 || !d_work ||
-        (bc != LPBF_BC_MIRROR && bc != LPBF_BC_DIRICHLET) ||
-        static_cast<long long>(k->nx) * k->ny >= (1LL << 31) - 1) {
+        (bc != LPBF_BC_MIRROR && bc != LPBF_BC_DIRICHLET)) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }
@@ -451,7 +457,8 @@ void lpbf_launch_step_f32(const float* theta_in, float* theta_out, int ld, const
 
 void lpbf_launch_f32_stats_finalize(const void* d_work, int nx, int ny, int depth, int rows,
                                     StepStats* d_stats, hipStream_t stream) {
-    if (!d_work || !d_stats || nx < 2 || ny < 1 || depth < 1 || depth > kMaxDepth || rows < 0) {
+    if (!d_work || !d_stats || !f32_shape_ok(nx, ny) || depth < 1 || depth > kMaxDepth ||
+        rows < 0) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }
@@ -463,7 +470,7 @@ void lpbf_launch_f32_stats_finalize(const void* d_work, int nx, int ny, int dept
 
 void lpbf_launch_noise_f32(float* theta, int nx, int ny, int ld, unsigned long long seed,
                            double base, double amp, hipStream_t stream) {
-    if (!theta || nx < 1 || ny < 1 || ld < nx) {
+    if (!theta || !f32_shape_ok(nx, ny) || ld < nx) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }
@@ -477,7 +484,7 @@ void lpbf_launch_noise_f32(float* theta, int nx, int ny, int ld, unsigned long l
 
 void lpbf_launch_theta_to_T(const float* theta, int ld, int nx, int ny, double T_ref, double* T,
                             hipStream_t stream) {
-    if (!theta || !T || nx < 1 || ny < 1 || ld < nx) {
+    if (!theta || !T || !f32_shape_ok(nx, ny) || ld < nx) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }
@@ -491,7 +498,7 @@ void lpbf_launch_theta_to_T(const float* theta, int ld, int nx, int ny, double T
 
 void lpbf_launch_T_to_theta(const double* T, int nx, int ny, double T_ref, float* theta, int ld,
                             hipStream_t stream) {
-    if (!theta || !T || nx < 1 || ny < 1 || ld < nx) {
+    if (!theta || !T || !f32_shape_ok(nx, ny) || ld < nx) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }

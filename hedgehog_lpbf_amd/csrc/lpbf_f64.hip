@@ -369,6 +369,10 @@ __global__ void k_noise_f64(double* __restrict__ T, long long n, unsigned long l
         T[i] = base + amp * 2.0 * noise_u(seed, static_cast<unsigned long long>(i));
 }
 
+// Largest field the FP64 kernels accept (cells): 128 GiB of FP64, and every
+// wavefront-tile count below stays far inside int range.
+constexpr long long kMaxCells = 1LL << 34;
+
 Region make_region(const double* in, double* out, int nx, int ny, int y_off, int bc,
                    double T_bc, double Tm, int ny_glob) {
     Region g;
@@ -378,6 +382,12 @@ Region make_region(const double* in, double* out, int nx, int ny, int y_off, int
     g.ntasks = g.nwx * ((ny + g.R - 1) / g.R);
     g.ny_glob = ny_glob;
     return g;
+}
+
+// Shape check shared by every launcher: the kernels trust nx and ny for
+// every address they form, so out-of-range values are rejected here.
+bool shape_ok(long long nx, long long ny) {
+    return nx >= 2 && ny >= 1 && nx <= (1LL << 30) && ny <= (1LL << 30) && nx * ny <= kMaxCells;
 }
 
 int grid_of(const Region& g) {
@@ -396,10 +406,7 @@ void launch_step(const Region& g, const StepConst& k, int phys, void* red, int f
     check_launch();
 }
 
-bool valid_grid(const StepConst* k) {
-    return k && k->nx >= 2 && k->ny >= 1 &&
-           static_cast<long long>(k->nx) * k->ny < (1LL << 40);
-}
+bool valid_grid(const StepConst* k) { return k && shape_ok(k->nx, k->ny); }
 
 }  // namespace
 }  // namespace lpbf_amd
@@ -431,7 +438,8 @@ void lpbf_launch_step_reduce(const double* Tin, double* Tout, const StepConst* k
 
 void lpbf_launch_reduce(const double* T, int nx, int ny, double Tm, Partial* out,
                         int nblocks, hipStream_t stream) {
-    if (!T || !out || nx < 1 || ny < 1 || nblocks < 1) {
+    if (!T || !out || nx < 1 || ny < 1 || static_cast<long long>(nx) * ny > kMaxCells ||
+        nblocks < 1 || nblocks > (1 << 24)) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }
@@ -441,7 +449,7 @@ void lpbf_launch_reduce(const double* T, int nx, int ny, double Tm, Partial* out
 
 void lpbf_launch_reduce_finalize(const Partial* parts, int nblocks, int nx, int ny,
                                  Partial* out, hipStream_t stream) {
-    if (!parts || !out || nblocks < 0) {
+    if (!parts || !out || nblocks < 0 || nx < 1 || ny < 1) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }
@@ -451,7 +459,8 @@ void lpbf_launch_reduce_finalize(const Partial* parts, int nblocks, int nx, int 
 
 void lpbf_launch_extract_col(const double* T, int nx, int ny, const Partial* glob,
                              double* out_col, hipStream_t stream) {
-    if (!T || !glob || !out_col || nx < 1 || ny < 1) {
+    if (!T || !glob || !out_col || nx < 1 || ny < 1 ||
+        static_cast<long long>(nx) * ny > kMaxCells) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }
@@ -463,8 +472,8 @@ void lpbf_launch_extract_col(const double* T, int nx, int ny, const Partial* glo
 
 void lpbf_launch_step_tile(const double* tile_in, double* tile_out, int owned_rows, int y0,
                            const StepConst* k, int phys, hipStream_t stream) {
-    if (!k || k->nx < 2 || owned_rows < 1 || !tile_in || !tile_out ||
-        (phys != 0 && phys != 1)) {
+    if (!k || !shape_ok(k->nx, static_cast<long long>(owned_rows) + 2) || owned_rows < 1 ||
+        y0 < 0 || !tile_in || !tile_out || (phys != 0 && phys != 1)) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }
@@ -474,7 +483,7 @@ void lpbf_launch_step_tile(const double* tile_in, double* tile_out, int owned_ro
 }
 
 int lpbf_step_ex_nblocks(int nx, int ny) {
-    if (nx < 1 || ny < 1) return 0;
+    if (!shape_ok(nx, ny)) return 0;
     const Region g = make_region(nullptr, nullptr, nx, ny, 0, BC_MIRROR, 0.0, 0.0, ny);
     return grid_of(g);
 }
@@ -495,7 +504,7 @@ void lpbf_launch_step_ex(const double* Tin, double* Tout, const StepConst* k, in
 
 void lpbf_launch_stats_finalize(const StepStats* parts, int nblocks, int nx, int ny,
                                 StepStats* out, hipStream_t stream) {
-    if (!parts || !out || nblocks < 0) {
+    if (!parts || !out || nblocks < 0 || nx < 1 || ny < 1) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }

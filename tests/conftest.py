@@ -27,3 +27,19 @@ _ensure_oracle()
 def ora():
     from oracle import pyoracle
     return pyoracle
+
+
+_GPU_ERROR_MARKERS = ("AcceleratorError", "hipError", "HIP error", "memory access fault",
+                      "illegal memory access", "No HIP GPUs")
+
+
+@pytest.hookimpl(hookwrapper=True)
+def pytest_runtest_makereport(item, call):
+    """Stop the session at the first GPU runtime error: after a device fault no
+    further GPU work is started in this process."""
+    outcome = yield
+    rep = outcome.get_result()
+    if rep.failed and call.excinfo is not None:
+        text = f"{call.excinfo.typename}: {call.excinfo.value}"
+        if any(m in text for m in _GPU_ERROR_MARKERS):
+            item.session.shouldstop = f"GPU runtime error in {item.nodeid}: {text[:200]}"

@@ -28,10 +28,6 @@ def kconst(k):
     return h.StepConst.from_buffer_copy(bytes(k))
 
 
-def addr(obj):
-    return C.addressof(obj)
-
-
 def plan_dev(plan: np.ndarray):
     return dev(np.ascontiguousarray(plan, dtype=np.float64))
 

@@ -38,16 +38,16 @@ def test_invalid_arguments_are_recorded():
     lib.lpbf_get_last_error()
     k = h.StepConst()
     k.nx, k.ny = 1, 4
-    lib.lpbf_launch_step(16, 32, C.addressof(k), 0, None)
+    lib.lpbf_launch_step(16, 32, k, 0, None)
     assert lib.lpbf_get_last_error() == h._lib.LPBF_ERR_INVALID_ARGUMENT
     assert lib.lpbf_get_last_error() == 0
     k.nx = 64
-    lib.lpbf_launch_step(16, 32, C.addressof(k), 7, None)        # bad phys
+    lib.lpbf_launch_step(16, 32, k, 7, None)        # bad phys
     assert lib.lpbf_get_last_error() == h._lib.LPBF_ERR_INVALID_ARGUMENT
-    lib.lpbf_launch_step_f32(16, 16, 64, C.addressof(k), 300.0, 0, 0.0, 1609.0, 16, 0, 1, 0,
+    lib.lpbf_launch_step_f32(16, 16, 64, k, 300.0, 0, 0.0, 1609.0, 16, 0, 1, 0,
                              None, 16, None)                      # in == out
     assert lib.lpbf_get_last_error() == h._lib.LPBF_ERR_INVALID_ARGUMENT
-    lib.lpbf_launch_step_f32(16, 48, 62, C.addressof(k), 300.0, 0, 0.0, 1609.0, 16, 0, 5, 0,
+    lib.lpbf_launch_step_f32(16, 48, 62, k, 300.0, 0, 0.0, 1609.0, 16, 0, 5, 0,
                              None, 16, None)                      # depth 5, ld < nx
     assert lib.lpbf_get_last_error() == h._lib.LPBF_ERR_INVALID_ARGUMENT
     assert "invalid" in lib.lpbf_error_string(1000).decode()

@@ -12,7 +12,7 @@ import pytest
 import torch
 
 import hedgehog_lpbf_amd as h
-from tests.devutil import addr, dev, dev_bytes, kconst, plan_dev, stats_np, stream, sync_check
+from tests.devutil import dev, dev_bytes, kconst, plan_dev, stats_np, stream, sync_check
 
 pytestmark = pytest.mark.gpu
 
@@ -38,7 +38,7 @@ def run_f32(k, plan, T0, n, depth, rows=0, bc=0, T_bc=300.0, Tm=1609.0, T_ref=30
         if return_prev and s + d == n:
             torch.cuda.synchronize()
             prev = a[:, :nx].cpu().numpy().copy()
-        lib.lpbf_launch_step_f32(a.data_ptr(), b.data_ptr(), ld, addr(kk), T_ref, bc, T_bc, Tm,
+        lib.lpbf_launch_step_f32(a.data_ptr(), b.data_ptr(), ld, kk, T_ref, bc, T_bc, Tm,
                                  pdev.data_ptr(), s, d, rows, stats.data_ptr() + 56 * s,
                                  work.data_ptr(), st)
         a, b = b, a
@@ -75,12 +75,17 @@ def test_depth_and_band_invariance(ora, nx, ny, bc):
     base, bst, _ = run_f32(k, plan, T0, n, 1, 0, bc)
     for depth, rows, pad in ((4, 0, 0), (3, 7, 1), (2, 33, 0), (4, 5, 3), (1, 3, 0)):
         th, sts, _ = run_f32(k, plan, T0, n, depth, rows, bc, pad=pad)
-        assert np.array_equal(th, base), (depth, rows)
-        for a_, b_ in zip(sts, bst):
+        diff = np.argwhere(th != base)
+        assert diff.size == 0, (depth, rows, len(diff), diff[:5].tolist())
+        # FP32 per-lane partial sums group cells by tile, so the energy sums
+        # may differ by FP32 rounding of the partials (~1e-6 K per cell)
+        tol = 1e-5 * nx * ny
+        for s_, (a_, b_) in enumerate(zip(sts, bst)):
             assert (a_.tmax, a_.arg, a_.ymin, a_.ymax, a_.molten) == \
-                   (b_.tmax, b_.arg, b_.ymin, b_.ymax, b_.molten)
+                   (b_.tmax, b_.arg, b_.ymin, b_.ymax, b_.molten), (depth, rows, s_)
             for f in ("sum_dT", "sum_src", "sum_loss"):
-                assert getattr(a_, f) == pytest.approx(getattr(b_, f), rel=1e-4, abs=1e-3)
+                assert abs(getattr(a_, f) - getattr(b_, f)) <= tol, \
+                    (depth, rows, s_, f, getattr(a_, f), getattr(b_, f))
 
 
 @pytest.mark.parametrize("depth", [1, 4])

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import hedgehog_lpbf_amd as h
-from tests.devutil import (addr, dev, dev_bytes, kconst, partials_np, stats_np, stream,
+from tests.devutil import (dev, dev_bytes, kconst, partials_np, stats_np, stream,
                            sync_check)
 
 pytestmark = pytest.mark.gpu
@@ -42,7 +42,7 @@ def test_step_single_parity(ora, nx, ny, phys):
     ref = ora.step(T, k, phys)
     a, b = dev(T), dev(np.zeros_like(T))
     kk = kconst(k)
-    h.lib().lpbf_launch_step(a.data_ptr(), b.data_ptr(), addr(kk), phys, stream())
+    h.lib().lpbf_launch_step(a.data_ptr(), b.data_ptr(), kk, phys, stream())
     sync_check()
     got = b.cpu().numpy()
     assert np.abs(got - ref).max() <= TOL
@@ -71,7 +71,7 @@ def test_case0_full_run_fused_path(ora):
     st = stream()
     for i in range(n):
         kk.x_l, kk.y_l = ticks[i].x_l, ticks[i].y_l
-        lib.lpbf_launch_step_reduce(a.data_ptr(), b.data_ptr(), addr(kk), 0, 1609.0,
+        lib.lpbf_launch_step_reduce(a.data_ptr(), b.data_ptr(), kk, 0, 1609.0,
                                     part.data_ptr(), st)
         a, b = b, a
         lib.lpbf_launch_reduce_finalize(part.data_ptr(), nf, nx, ny, glob.data_ptr() + 24 * i, st)
@@ -125,7 +125,7 @@ def test_step_reduce_fills_reference_partial_count(ora, nx, ny):
     a, b = dev(T), dev(np.zeros_like(T))
     out = dev_bytes(24)
     st = stream()
-    h.lib().lpbf_launch_step_reduce(a.data_ptr(), b.data_ptr(), addr(kconst(k)), 0, 1609.0,
+    h.lib().lpbf_launch_step_reduce(a.data_ptr(), b.data_ptr(), kconst(k), 0, 1609.0,
                                     part.data_ptr(), st)
     h.lib().lpbf_launch_reduce_finalize(part.data_ptr(), nf, nx, ny, out.data_ptr(), st)
     sync_check()
@@ -150,7 +150,7 @@ def test_step_tile_matches_oracle_tiles(ora):
             ref = ora.step_tile(tile, y1 - y0 + 1, y0, k, phys)
             a, b = dev(tile), dev(np.zeros_like(tile))
             h.lib().lpbf_launch_step_tile(a.data_ptr(), b.data_ptr(), y1 - y0 + 1, y0,
-                                          addr(kconst(k)), phys, stream())
+                                          kconst(k), phys, stream())
             sync_check()
             got = b.cpu().numpy()
             assert np.abs(got - ref).max() <= TOL
@@ -171,7 +171,7 @@ def test_step_ex_stats(ora, bc):
     out = dev_bytes(56)
     a, b = dev(T), dev(np.zeros_like(T))
     st = stream()
-    h.lib().lpbf_launch_step_ex(a.data_ptr(), b.data_ptr(), addr(kconst(k)), 0, bc, 300.0,
+    h.lib().lpbf_launch_step_ex(a.data_ptr(), b.data_ptr(), kconst(k), 0, bc, 300.0,
                                 1609.0, blk.data_ptr(), st)
     h.lib().lpbf_launch_stats_finalize(blk.data_ptr(), nb, nx, ny, out.data_ptr(), st)
     sync_check()
@@ -206,7 +206,7 @@ def test_config1_dirichlet_512_full_run(ora):
     st = stream()
     for i in range(n):
         kk.x_l, kk.y_l, kk.q_scale = plan[i]
-        h.lib().lpbf_launch_step_ex(a.data_ptr(), b.data_ptr(), addr(kk), 0, 1, 300.0, 1609.0,
+        h.lib().lpbf_launch_step_ex(a.data_ptr(), b.data_ptr(), kk, 0, 1, 300.0, 1609.0,
                                     blk.data_ptr(), st)
         h.lib().lpbf_launch_stats_finalize(blk.data_ptr(), nb, 512, 512,
                                            sts.data_ptr() + 56 * i, st)
@@ -234,6 +234,6 @@ def test_invalid_argument_reported_without_launch():
     h.lib().lpbf_get_last_error()
     k = h.StepConst()
     k.nx, k.ny = 1, 5
-    h.lib().lpbf_launch_step(16, 32, addr(k), 0, None)
+    h.lib().lpbf_launch_step(16, 32, k, 0, None)
     with pytest.raises(h.LpbfError):
         h.check()
