@@ -76,12 +76,37 @@ def traffic_for(workload):
     return None
 
 
+def copy_reference(args):
+    """Plain device-to-device copy of an n x n FP32 plate, `steps` times."""
+    import torch
+    n = args.n
+    a = torch.ones((n, n), dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / max(args.steps, 1)
+    nbytes = 2 * a.numel() * 4
+    print(json.dumps({"metric": "copy bytes/s", "value": nbytes / (ms * 1e-3),
+                      "unit": "B/s", "bytes_per_copy": nbytes, "read_bytes": nbytes // 2,
+                      "ms_per_copy": ms, "steps": args.steps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="plate8k_f32", choices=["plate8k_f32", "plate8k_f64"])
+    ap.add_argument("--workload", default="plate8k_f32",
+                    choices=["plate8k_f32", "plate8k_f64", "copy"],
+                    help="copy: a plain device copy of one n x n FP32 plate per step, the "
+                         "reference for calibrating HBM byte counters")
     ap.add_argument("--depth", type=int, default=4)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--n", type=int, default=8192, help="plate side (cells)")
@@ -98,6 +123,8 @@ def main():
     import hedgehog_lpbf_amd as h
 
     lib = h.lib()
+    if args.workload == "copy":
+        return copy_reference(args)
     mat, pr, dom, dt, k, plan = plate_setup(args.n, args.n)
     nx, ny = dom.nx, dom.ny
     cells = nx * ny

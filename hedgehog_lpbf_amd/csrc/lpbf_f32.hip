@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f32(F32Args a,
         const int yb = min(ya + a.R, ny);
         const int y_lo = max(ya - K, 0);
         const int y_hi = min(yb + K, ny);
-        const int i_end = (yb == ny) ? ny - 1 + K : y_hi - 1;
+        const int i_end = yb - 1 + K;   // level-K row yb-1 is produced at i = yb-1+K
         // edge columns held by this strip
         const bool has_left = (L0 < 0);                  // column -1 at lane 0, elem 3
         const int pR = nx - L0;                          // position of column nx
