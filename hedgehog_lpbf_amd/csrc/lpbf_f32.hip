@@ -23,6 +23,9 @@
 //     consecutive tiles land on the same XCD (shared halo cache lines in L2).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <type_traits>
+
 #include "lpbf_common.hpp"
 #include "lpbf_kernels.h"
 
@@ -44,20 +47,18 @@ struct F32Args {
     int nx, ny, ld;
     int nstrips, R, ntasks, nwg;
     int bc;
-    float cx, cy;        // alpha*dt/dx^2, alpha*dt/dy^2
-    float cHB, c0;       // linear loss per step: cHB*theta + c0
-    float cR;            // radiation loss per step / (T^4 - Ta^4); 0 if off
-    float dA;            // T_ref - Tamb
-    float cTT;           // T_ref + Tamb
-    float Tref;
-    float Ta2;           // Tamb^2
-    float theta_m;       // Tm - T_ref
-    float theta_bc;      // T_bc - T_ref
-    float ax, ay;        // log2(e) * 2*dx^2/r0^2, log2(e) * 2*dy^2/r0^2
-    float cut;           // exponent (base 2) beyond which the source is dropped
+    float cx, cy;          // alpha*dt/dx^2, alpha*dt/dy^2
+    // dt*loss(theta) in K = (((a4*th + a3)*th + a2)*th + a1)*th + a0 (Horner):
+    // convection + substrate are linear in theta, radiation expands
+    // (T_ref + theta)^4 - Tamb^4 around T_ref.
+    float a0, a1, a2, a3, a4;
+    float theta_m;         // Tm - T_ref
+    float theta_bc;        // T_bc - T_ref
+    float ax, ay;          // log2(e) * 2*dx^2/r0^2, log2(e) * 2*dy^2/r0^2
+    float cut;             // base-2 exponent beyond which the source is dropped
     double inv_dx, inv_dy, cS0;  // 1/dx, 1/dy, dt*peak_S/(rho_cp*h)
     double Tref_d;
-    StepStats* part;     // [depth][nwg]
+    StepStats* part;       // [depth][nwg]
 };
 
 __device__ inline int xcd_remap(int bid, int n) {
@@ -66,36 +67,64 @@ __device__ inline int xcd_remap(int bid, int n) {
     return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
+// Whole-wavefront lane shifts (DPP wave_shr:1 / wave_shl:1, GFX9 family):
+// lane l receives lane l-1 (from_left) or lane l+1 (from_right); the lane
+// without a source receives 0.
+__device__ __forceinline__ float from_left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float from_right(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+
+__device__ __forceinline__ float& elem(float4& v, int e) {
+    return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w));
+}
+
 struct LevelAcc {
-    float bv;       // best theta
-    int bi;         // its row-major index
-    int ymin, ymax;
+    float bv;        // best theta of this lane
+    int bi;          // its row-major index (smallest on ties)
+    int ymin, ymax;  // molten rows
     int molten;
-    float sd, ss, sl;
+    double S;        // sum of theta over the owned cells after this step
+    float SL;        // sum of dt*loss
+    float SS;        // sum of dt*S
 };
 
-template <int K, int BC>
+template <int K, int BC, bool RAD>
 __global__ __launch_bounds__(kThreads) void k_step_f32(F32Args a,
                                                        const LaserTick* __restrict__ plan,
                                                        int step0) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
+    // wave-uniform: everything derived from the tile index lives in SGPRs
+    // and the row/strip conditions below are scalar branches
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int wg = xcd_remap(blockIdx.x, a.nwg);
     const int task = wg * kWaves + wave;
     const int nx = a.nx, ny = a.ny;
+    constexpr float kNegInf = -3.0e38f;
 
+    // per level: laser position in cell units, source scale, and the rows
+    // [srlo, srhi] whose source can matter (|r - yc| below the cut-off, plus
+    // one row of margin); the row test is then a scalar compare
     float xc[K], yc[K], cS[K];
+    int srlo[K], srhi[K];
+    const float wcut = sqrtf(a.cut / a.ay);
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         const LaserTick t = plan[step0 + j];
         xc[j] = static_cast<float>(t.x_l * a.inv_dx);
         yc[j] = static_cast<float>(t.y_l * a.inv_dy);
         cS[j] = static_cast<float>(a.cS0 * t.q_scale);
+        srlo[j] = __builtin_amdgcn_readfirstlane(static_cast<int>(floorf(yc[j] - wcut)) - 1);
+        srhi[j] = __builtin_amdgcn_readfirstlane(static_cast<int>(ceilf(yc[j] + wcut)) + 1);
+        if (t.q_scale == 0.0) { srlo[j] = 1; srhi[j] = 0; }   // beam off: no source rows
     }
 
     LevelAcc acc[K];
 #pragma unroll
-    for (int j = 0; j < K; ++j) acc[j] = LevelAcc{-3.0e38f, 0x7fffffff, ny, -1, 0, 0.f, 0.f, 0.f};
+    for (int j = 0; j < K; ++j) acc[j] = LevelAcc{kNegInf, 0x7fffffff, ny, -1, 0, 0.0, 0.f, 0.f};
+    double S0 = 0.0;   // sum of the input theta over the owned cells
 
     if (task < a.ntasks) {
         const int band = task / a.nstrips, strip = task % a.nstrips;
@@ -103,15 +132,11 @@ __global__ __launch_bounds__(kThreads) void k_step_f32(F32Args a,
         const int col4 = L0 + 4 * lane;
         const bool ld_ok = col4 >= 0 && col4 < nx;
         const bool own_lane = lane >= 1 && lane <= kWave - 2 && col4 < nx;
-        bool own_e[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) own_e[e] = own_lane && (col4 + e < nx);
         const int ya = band * a.R;
         const int yb = min(ya + a.R, ny);
         const int y_lo = max(ya - K, 0);
         const int y_hi = min(yb + K, ny);
         const int i_end = yb - 1 + K;   // level-K row yb-1 is produced at i = yb-1+K
-        // edge columns held by this strip
         const bool has_left = (L0 < 0);                  // column -1 at lane 0, elem 3
         const int pR = nx - L0;                          // position of column nx
         const bool has_right = (pR >= 0 && pR < kLoadCols);
@@ -119,16 +144,17 @@ __global__ __launch_bounds__(kThreads) void k_step_f32(F32Args a,
         const int pL = nx - 1 - L0;                      // position of column nx-1
         const bool has_last = (pL >= 0 && pL < kLoadCols);
         const int lastl = pL >> 2, laste = pL & 3;
+        // the lane that holds column nx-1 may also hold columns >= nx
+        const bool ragged = has_right && pade != 0;
 
         // Mirror: column -1 := column 0, column nx := column nx-1.
-        // Dirichlet: column 0 and nx-1 := theta_bc (levels >= 1).
         auto fix_mirror = [&](float4& v) {
             if (has_left) {
-                const float t = __shfl_down(v.x, 1);
+                const float t = from_right(v.x);
                 if (lane == 0) v.w = t;
             }
             if (has_right) {
-                const float t = __shfl_up(v.w, 1);
+                const float t = from_left(v.w);
                 if (lane == padl) {
                     if (pade == 0) v.x = t;
                     else if (pade == 1) v.y = v.x;
@@ -137,129 +163,213 @@ __global__ __launch_bounds__(kThreads) void k_step_f32(F32Args a,
                 }
             }
         };
-        auto fix_dirichlet = [&](float4& v) {
-            if (has_left && lane == 1) v.x = a.theta_bc;   // column 0 at position 4
+        // Dirichlet: columns 0 and nx-1 := theta_bc; their loss leaves the sums.
+        auto fix_dirichlet = [&](float4& v, float4& L) {
+            if (has_left && lane == 1) { v.x = a.theta_bc; L.x = 0.f; }   // column 0
             if (has_last && lane == lastl) {
-                if (laste == 0) v.x = a.theta_bc;
-                else if (laste == 1) v.y = a.theta_bc;
-                else if (laste == 2) v.z = a.theta_bc;
-                else v.w = a.theta_bc;
+                elem(v, laste) = a.theta_bc;
+                elem(L, laste) = 0.f;
             }
         };
-
-        float4 W[K][3];
+        // Owned-cell view of a row for the reductions: non-owned lanes and
+        // columns >= nx drop out (-inf for the max, 0 for the sums).
+        auto row_sum = [&](float4 v) {
+            if (ragged && lane == lastl) {
 #pragma unroll
-        for (int j = 0; j < K; ++j)
-#pragma unroll
-            for (int q = 0; q < 3; ++q) W[j][q] = make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int e = 0; e < 4; ++e)
+                    if (e > laste) elem(v, e) = 0.f;
+            }
+            const float s = (v.x + v.y) + (v.z + v.w);
+            return own_lane ? s : 0.f;
+        };
 
-        // rows are prefetched kPf iterations ahead of their use
+        auto own_sum = [&](float4 v) {
+            const float t = (v.x + v.y) + (v.z + v.w);
+            return own_lane ? t : 0.f;
+        };
+
         auto load_row = [&](int i) {
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
             if (i < y_hi && ld_ok)
                 v = *reinterpret_cast<const float4*>(a.in + static_cast<long long>(i) * a.ld + col4);
             return v;
         };
-        float4 pf[kPf];
-#pragma unroll
-        for (int q = 0; q < kPf; ++q) pf[q] = load_row(y_lo + q);
+        // interior strips only (every lane in bounds): rows past y_hi are
+        // never consumed, so the row index is clamped instead of tested
+        auto load_row_interior = [&](int i) {
+            const int ii = min(i, y_hi - 1);
+            return *reinterpret_cast<const float4*>(a.in + static_cast<long long>(ii) * a.ld + col4);
+        };
 
-        for (int i = y_lo; i <= i_end; ++i) {
-            float4 v = pf[0];
+        // Three-slot ring per level: at ring phase ph the newest row sits in
+        // slot ph, the one before in (ph+2)%3 and the oldest in (ph+1)%3, so
+        // the window advances without register moves.  The general body
+        // always runs at phase 0 and rotates afterwards; the steady-state
+        // body runs the three phases back to back.
+        float4 W[K][3];
+        float4 PF[3];
 #pragma unroll
-            for (int q = 0; q < kPf - 1; ++q) pf[q] = pf[q + 1];
-            pf[kPf - 1] = load_row(i + kPf);
-            if (BC == LPBF_BC_MIRROR && i < y_hi) fix_mirror(v);
-            W[0][0] = W[0][1]; W[0][1] = W[0][2]; W[0][2] = v;
+        for (int q = 0; q < 3; ++q) PF[q] = load_row(y_lo + q);
 
-#pragma unroll
-            for (int j = 1; j <= K; ++j) {
-                const int r = i - j;
+        // One level-j row from level j-1's window.  GEN: any row of any strip
+        // (edge columns, edge rows, band-halo rows, mirror rows); otherwise
+        // an interior strip with every level on an owned interior row.
+        auto level = [&](auto PHC, auto JC, auto GENC, int i) {
+            constexpr int ph = decltype(PHC)::value;
+            constexpr int j = decltype(JC)::value;
+            constexpr bool GEN = decltype(GENC)::value;
+            constexpr int s_new = ph, s_mid = (ph + 2) % 3, s_old = (ph + 1) % 3;
+            const int r = i - j;
+            if (GEN) {
                 const int lo = max(ya - (K - j), 0);
                 const int hi = min(yb + (K - j), ny);
-                float4 res = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (r >= lo && r < hi) {
-                    const float4 c = W[j - 1][1];
-                    const float4 up = (r == 0) ? c : W[j - 1][0];
-                    const float4 dn = (r == ny - 1) ? c : W[j - 1][2];
-                    const float wl = __shfl_up(c.w, 1);    // column left of c.x
-                    const float wr = __shfl_down(c.x, 1);  // column right of c.w
-                    const float cv[4] = {c.x, c.y, c.z, c.w};
-                    const float xm[4] = {wl, c.x, c.y, c.z};
-                    const float xp[4] = {c.y, c.z, c.w, wr};
-                    const float uv[4] = {up.x, up.y, up.z, up.w};
-                    const float dv[4] = {dn.x, dn.y, dn.z, dn.w};
-                    // source: the whole row is skipped unless it is near the laser
-                    const float uy = static_cast<float>(r) - yc[j - 1];
-                    const float ey = a.ay * uy * uy;
-                    const bool src_row = ey < a.cut;
-                    float out4[4], srcv[4], lossv[4];
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const float th = cv[e];
-                        float s = 0.f;
-                        if (src_row) {
-                            const float ux = static_cast<float>(col4 + e) - xc[j - 1];
-                            s = cS[j - 1] * exp2f(-(a.ax * ux * ux + ey));
-                        }
-                        const float Tt = th + a.Tref;
-                        float loss = a.cHB * th + a.c0;
-                        if (a.cR != 0.f) {
-                            const float rad = (th + a.dA) * (th + a.cTT) * (Tt * Tt + a.Ta2);
-                            loss += a.cR * rad;
-                        }
-                        const float lap = a.cx * ((xm[e] + xp[e]) - 2.f * th)
-                                        + a.cy * ((uv[e] + dv[e]) - 2.f * th);
-                        out4[e] = th + ((lap + s) - loss);
-                        srcv[e] = s;
-                        lossv[e] = loss;
-                    }
-                    res = make_float4(out4[0], out4[1], out4[2], out4[3]);
-                    if (BC == LPBF_BC_DIRICHLET) {
-                        if (r == 0 || r == ny - 1)
-                            res = make_float4(a.theta_bc, a.theta_bc, a.theta_bc, a.theta_bc);
-                        else
-                            fix_dirichlet(res);
-                    } else {
-                        fix_mirror(res);
-                    }
-                    if (r >= ya && r < yb) {
-                        if (j == K && own_lane)
-                            *reinterpret_cast<float4*>(a.out + static_cast<long long>(r) * a.ld + col4) = res;
-                        const float rv[4] = {res.x, res.y, res.z, res.w};
-                        LevelAcc& A = acc[j - 1];
-                        const bool upd_row = !(BC == LPBF_BC_DIRICHLET && (r == 0 || r == ny - 1));
-                        int m = 0;
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) {
-                            if (!own_e[e]) continue;
-                            if (rv[e] > A.bv) { A.bv = rv[e]; A.bi = r * nx + col4 + e; }
-                            m += (rv[e] > a.theta_m) ? 1 : 0;
-                            const bool upd = upd_row &&
-                                !(BC == LPBF_BC_DIRICHLET && (col4 + e == 0 || col4 + e == nx - 1));
-                            if (upd) {
-                                A.sd += rv[e] - cv[e];
-                                A.ss += srcv[e];
-                                A.sl += lossv[e];
-                            }
-                        }
-                        if (m) {
-                            A.molten += m;
-                            if (r < A.ymin) A.ymin = r;
-                            A.ymax = r;
-                        }
-                    }
+                if (r < lo || r >= hi) {
+                    if (j < K && r == ny) W[j][s_new] = W[j][s_mid];   // row ny := row ny-1
+                    return;
                 }
-                if (j < K) { W[j][0] = W[j][1]; W[j][1] = W[j][2]; W[j][2] = res; }
             }
+            const float4 c = W[j - 1][s_mid];
+            const float4 up = W[j - 1][s_old];
+            const float4 dn = W[j - 1][s_new];
+            const float wl = from_left(c.w);    // column left of c.x
+            const float wr = from_right(c.x);   // column right of c.w
+            const float cv[4] = {c.x, c.y, c.z, c.w};
+            const float xm[4] = {wl, c.x, c.y, c.z};
+            const float xp[4] = {c.y, c.z, c.w, wr};
+            const float uv[4] = {up.x, up.y, up.z, up.w};
+            const float dv[4] = {dn.x, dn.y, dn.z, dn.w};
+            float out4[4], loss4[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float th = cv[e];
+                float L;
+                if (RAD) L = fmaf(fmaf(fmaf(fmaf(a.a4, th, a.a3), th, a.a2), th, a.a1), th, a.a0);
+                else L = fmaf(a.a1, th, a.a0);
+                float d = fmaf(a.cx, fmaf(-2.f, th, xm[e] + xp[e]), -L);
+                d = fmaf(a.cy, fmaf(-2.f, th, uv[e] + dv[e]), d);
+                out4[e] = d;
+                loss4[e] = L;
+            }
+            // source: the whole row is skipped unless it is near the laser
+            float ssum = 0.f;
+            if (r >= srlo[j - 1] && r <= srhi[j - 1]) {
+                const float uy = static_cast<float>(r) - yc[j - 1];
+                const float ey = a.ay * uy * uy;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float ux = static_cast<float>(col4 + e) - xc[j - 1];
+                    // v_exp_f32: results below 2^-126 flush to 0, far under
+                    // FP32 resolution of any theta
+                    const float sv = cS[j - 1] * __builtin_amdgcn_exp2f(-fmaf(a.ax * ux, ux, ey));
+                    out4[e] += sv;
+                    ssum += (col4 + e < nx) ? sv : 0.f;
+                }
+                ssum = own_lane ? ssum : 0.f;
+            }
+            float4 res = make_float4(cv[0] + out4[0], cv[1] + out4[1], cv[2] + out4[2],
+                                     cv[3] + out4[3]);
+            float4 L4 = make_float4(loss4[0], loss4[1], loss4[2], loss4[3]);
+            if (GEN) {
+                if (BC == LPBF_BC_DIRICHLET) {
+                    if (r == 0 || r == ny - 1) {
+                        res = make_float4(a.theta_bc, a.theta_bc, a.theta_bc, a.theta_bc);
+                        L4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                        ssum = 0.f;
+                    } else {
+                        fix_dirichlet(res, L4);
+                    }
+                } else {
+                    fix_mirror(res);
+                }
+            }
+            if (!GEN || (r >= ya && r < yb)) {
+                if (j == K && own_lane)
+                    *reinterpret_cast<float4*>(a.out + static_cast<long long>(r) * a.ld + col4) = res;
+                LevelAcc& A = acc[j - 1];
+                float4 mx = res;
+                if (GEN && ragged && lane == lastl) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (e > laste) elem(mx, e) = kNegInf;
+                }
+                float m4 = fmaxf(fmaxf(mx.x, mx.y), fmaxf(mx.z, mx.w));
+                m4 = own_lane ? m4 : kNegInf;
+                if (m4 > A.bv) {
+                    A.bv = m4;
+                    const int e = (mx.x == m4) ? 0 : (mx.y == m4) ? 1 : (mx.z == m4) ? 2 : 3;
+                    A.bi = r * nx + col4 + e;
+                }
+                if (m4 > a.theta_m) {
+                    A.molten += (mx.x > a.theta_m) + (mx.y > a.theta_m) + (mx.z > a.theta_m) +
+                                (mx.w > a.theta_m);
+                    A.ymin = min(A.ymin, r);
+                    A.ymax = r;
+                }
+                A.S += static_cast<double>(GEN ? row_sum(res) : own_sum(res));
+                A.SL += GEN ? row_sum(L4) : own_sum(L4);
+                A.SS += ssum;
+            }
+            if (j < K) {
+                W[j][s_new] = res;
+                if (GEN && r == 0) W[j][s_mid] = res;   // row -1 := row 0
+            }
+        };
+
+        auto body = [&](auto PHC, auto GENC, int i) {
+            constexpr int ph = decltype(PHC)::value;
+            constexpr bool GEN = decltype(GENC)::value;
+            constexpr int s_new = ph, s_mid = (ph + 2) % 3;
+            float4 v = PF[ph];
+            PF[ph] = GEN ? load_row(i + 3) : load_row_interior(i + 3);
+            if (!GEN || i < y_hi) {
+                if (GEN && BC == LPBF_BC_MIRROR) fix_mirror(v);
+                if (i >= ya && i < yb)
+                    S0 += static_cast<double>(GEN ? row_sum(v) : own_sum(v));
+            } else if (i == ny) {
+                v = W[0][s_mid];                 // row ny := row ny-1
+            }
+            W[0][s_new] = v;
+            if (GEN && i == 0) W[0][s_mid] = v;  // row -1 := row 0
+            level(PHC, std::integral_constant<int, 1>{}, GENC, i);
+            if (K >= 2) level(PHC, std::integral_constant<int, (K >= 2 ? 2 : 1)>{}, GENC, i);
+            if (K >= 3) level(PHC, std::integral_constant<int, (K >= 3 ? 3 : 1)>{}, GENC, i);
+            if (K >= 4) level(PHC, std::integral_constant<int, (K >= 4 ? 4 : 1)>{}, GENC, i);
+        };
+        using P0 = std::integral_constant<int, 0>;
+        using P1 = std::integral_constant<int, 1>;
+        using P2 = std::integral_constant<int, 2>;
+        using Gen = std::true_type;
+        using Fast = std::false_type;
+        auto general = [&](int i) {
+            body(P0{}, Gen{}, i);
+            // back to the phase-0 layout: oldest <- previous, previous <- newest
+#pragma unroll
+            for (int q = 0; q < K; ++q) { W[q][1] = W[q][2]; W[q][2] = W[q][0]; }
+            const float4 t = PF[0];
+            PF[0] = PF[1]; PF[1] = PF[2]; PF[2] = t;
+        };
+
+        // steady state: interior strip, every level on an owned row that is
+        // neither an edge row nor next to one
+        const bool interior = !has_left && !has_right && !has_last;
+        const int i_s = interior ? max(ya + K, K + 1) : i_end + 1;
+        const int i_e = interior ? min(yb + 1, ny - 1) : i_end + 1;
+        int i = y_lo;
+        for (; i <= i_end && i < i_s; ++i) general(i);
+        for (; i + 3 <= i_e; i += 3) {
+            body(P0{}, Fast{}, i);
+            body(P1{}, Fast{}, i + 1);
+            body(P2{}, Fast{}, i + 2);
         }
+        for (; i <= i_end; ++i) general(i);
     }
 
     // per level: wavefront then workgroup merge -> a.part[j*nwg + wg]
     __shared__ PartialAcc s_pa[kWaves];
-    __shared__ double s_sum[3][kWaves];
+    __shared__ double s_sum[4][kWaves];
     __shared__ long long s_m[kWaves];
     const long long N = static_cast<long long>(nx) * ny;
+    double prevS = wave_sum(S0);   // lane 0 holds the wavefront sums
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         const LevelAcc& A = acc[j];
@@ -269,14 +379,18 @@ __global__ __launch_bounds__(kThreads) void k_step_f32(F32Args a,
         p.ymin = A.ymin;
         p.ymax = A.ymax;
         const PartialAcc w = wave_reduce(p);
-        const double wd = wave_sum(static_cast<double>(A.sd));
-        const double ws = wave_sum(static_cast<double>(A.ss));
-        const double wl = wave_sum(static_cast<double>(A.sl));
+        const double wS = wave_sum(A.S);
+        const double ws = wave_sum(static_cast<double>(A.SS));
+        const double wl = wave_sum(static_cast<double>(A.SL));
         const long long wm = wave_sum(static_cast<long long>(A.molten));
         if (lane == 0) {
-            s_pa[wave] = w; s_sum[0][wave] = wd; s_sum[1][wave] = ws; s_sum[2][wave] = wl;
+            s_pa[wave] = w;
+            s_sum[0][wave] = wS - prevS;   // sum of (theta' - theta) over the owned cells
+            s_sum[1][wave] = ws;
+            s_sum[2][wave] = wl;
             s_m[wave] = wm;
         }
+        prevS = wS;
         __syncthreads();
         if (threadIdx.x == 0) {
             PartialAcc b = s_pa[0];
@@ -334,13 +448,38 @@ __global__ void k_T_to_theta(const double* __restrict__ T, int nx, int ny, doubl
     }
 }
 
-int rows_per_band(int nx, int ny, int depth) {
-    // enough wavefront tiles to fill 256 CUs several times, long enough
-    // bands that the 2*depth recomputed halo rows stay a small fraction
+// Band height: minimise the rows one SIMD streams, ceil(tiles/1024) *
+// (R + 2*depth), scaled by how well that many resident wavefronts per SIMD
+// hide latency (1: 0.6, 2: 0.85, >= 3: 1); the halo rows are recomputed
+// work, the tile count sets the tail and the occupancy.
+int rows_per_band_search(int nx, int ny, int depth) {
     const int nstrips = (nx + kOutCols - 1) / kOutCols;
-    int R = (depth > 1) ? 128 : 64;
-    while (R > 16 && static_cast<long long>(nstrips) * ((ny + R - 1) / R) < 4096) R /= 2;
-    if (depth > 1 && R < 8 * depth) R = 8 * depth;
+    const int lo = depth > 1 ? 8 * depth : 16;
+    int best = lo;
+    double best_cost = 1e300;
+    for (int R = lo; R <= ny + lo; ++R) {
+        const long long tiles = static_cast<long long>(nstrips) * ((ny + R - 1) / R);
+        const long long per = (tiles + 1023) / 1024;
+        const double eff = per >= 3 ? 1.0 : (per == 2 ? 0.85 : 0.6);
+        const double cost = static_cast<double>(per) * (R + 2 * depth) / eff;
+        if (cost < best_cost - 1e-9) { best_cost = cost; best = R; }
+        if (R >= ny) break;
+    }
+    return best;
+}
+
+int rows_per_band(int nx, int ny, int depth) {
+    // the search is O(ny); remember the last few shapes
+    struct Entry { int nx, ny, depth, R; };
+    static std::mutex mu;
+    static Entry cache[16];
+    static int next = 0;
+    std::lock_guard<std::mutex> g(mu);
+    for (const Entry& e : cache)
+        if (e.nx == nx && e.ny == ny && e.depth == depth && e.R > 0) return e.R;
+    const int R = rows_per_band_search(nx, ny, depth);
+    cache[next] = {nx, ny, depth, R};
+    next = (next + 1) % 16;
     return R;
 }
 
@@ -357,14 +496,20 @@ Geometry geometry(int nx, int ny, int depth, int rows) {
     return g;
 }
 
+template <int K, bool RAD>
+void launch_kr(const F32Args& a, const LaserTick* plan, int step0, hipStream_t s) {
+    if (a.bc == LPBF_BC_DIRICHLET)
+        hipLaunchKernelGGL((k_step_f32<K, LPBF_BC_DIRICHLET, RAD>), dim3(a.nwg), dim3(kThreads), 0,
+                           s, a, plan, step0);
+    else
+        hipLaunchKernelGGL((k_step_f32<K, LPBF_BC_MIRROR, RAD>), dim3(a.nwg), dim3(kThreads), 0, s,
+                           a, plan, step0);
+}
+
 template <int K>
 void launch_k(const F32Args& a, const LaserTick* plan, int step0, hipStream_t s) {
-    if (a.bc == LPBF_BC_DIRICHLET)
-        hipLaunchKernelGGL((k_step_f32<K, LPBF_BC_DIRICHLET>), dim3(a.nwg), dim3(kThreads), 0, s, a,
-                           plan, step0);
-    else
-        hipLaunchKernelGGL((k_step_f32<K, LPBF_BC_MIRROR>), dim3(a.nwg), dim3(kThreads), 0, s, a,
-                           plan, step0);
+    if (a.a4 != 0.f) launch_kr<K, true>(a, plan, step0, s);
+    else launch_kr<K, false>(a, plan, step0, s);
 }
 
 // The FP32 kernels index cells with 32-bit ints and rely on nx, ny for every
@@ -384,16 +529,19 @@ F32Args make_args(const float* in, float* out, int ld, const StepConst& k, doubl
     const double dt = k.dt;
     a.cx = static_cast<float>(k.alpha * dt * k.inv_dx2);
     a.cy = static_cast<float>(k.alpha * dt * k.inv_dy2);
-    // loss*dt = dt*((hConv*(T-Tamb) + (K/dSub)*(T-Tsub))*inv_h/rho_cp + rad)
+    // dt*loss in K as a polynomial in theta = T - T_ref:
+    //   cH*(theta + T_ref - Tamb) + cB*(theta + T_ref - Tsub)
+    //   + cR*((T_ref + theta)^4 - Tamb^4)
     const double cH = dt * k.hConv * k.inv_h / k.rho_cp;
     const double cB = dt * (k.K / k.dSub) * k.inv_h / k.rho_cp;
-    a.cHB = static_cast<float>(cH + cB);
-    a.c0 = static_cast<float>(cH * (T_ref - k.Tamb) + cB * (T_ref - k.Tsub));
-    a.cR = k.radiation ? static_cast<float>(dt * k.eps * 5.670374419e-8 * k.inv_h / k.rho_cp) : 0.f;
-    a.dA = static_cast<float>(T_ref - k.Tamb);
-    a.cTT = static_cast<float>(T_ref + k.Tamb);
-    a.Tref = static_cast<float>(T_ref);
-    a.Ta2 = static_cast<float>(k.Tamb * k.Tamb);
+    const double cR = k.radiation ? dt * k.eps * 5.670374419e-8 * k.inv_h / k.rho_cp : 0.0;
+    const double Tr = T_ref, Ta = k.Tamb;
+    a.a0 = static_cast<float>(cH * (Tr - Ta) + cB * (Tr - k.Tsub) +
+                              cR * (Tr * Tr * Tr * Tr - Ta * Ta * Ta * Ta));
+    a.a1 = static_cast<float>(cH + cB + 4.0 * cR * Tr * Tr * Tr);
+    a.a2 = static_cast<float>(6.0 * cR * Tr * Tr);
+    a.a3 = static_cast<float>(4.0 * cR * Tr);
+    a.a4 = static_cast<float>(cR);
     a.theta_m = static_cast<float>(Tm - T_ref);
     a.theta_bc = static_cast<float>(T_bc - T_ref);
     const double log2e = 1.4426950408889634;

@@ -108,6 +108,30 @@ def test_error_against_fp64_oracle(ora, depth):
     assert abs(sts[-1].molten - rst[-1].molten) <= 3
 
 
+@pytest.mark.parametrize("depth,rows", [(3, 0), (4, 37)])
+def test_error_against_fp64_oracle_many_strips(ora, depth, rows):
+    """1100x150 (five 248-column strips, three of them interior) with the
+    laser crossing strip boundaries: FP32 within 0.05 K of the FP64 oracle."""
+    print(f"\n[case] f32 vs fp64 wide depth={depth} rows={rows}", flush=True)
+    nx, ny, n = 1100, 150, 240
+    s = _setup(ora, nx, ny)
+    s.proc.Tamb = s.proc.Tsub = 300.0
+    s.proc.V = 4.0
+    k = ora.make_const(s, ora.dt_cfl(s))
+    plan = ora.track_plan(s.proc.V, k.dt, s.dom.Lx, s.dom.Ly, n)
+    T0 = 300.0 + ora.noise_field(3, 0.0, 0.5, (ny, nx))
+    Tref, rst = ora.run_plan(k, plan, T0)
+    th, sts, _ = run_f32(k, plan, T0, n, depth, rows)
+    err = np.abs(300.0 + th.astype(np.float64) - Tref).max()
+    print(f"max|dT| f32 vs f64 = {err:.3e} K, peak {Tref.max():.1f} K")
+    assert err <= 0.05
+    for i in (0, n // 2, n - 1):
+        assert abs(sts[i].tmax - rst[i].tmax) <= 0.05
+        assert sts[i].sum_src == pytest.approx(rst[i].sum_src, rel=1e-4)
+        assert sts[i].sum_loss == pytest.approx(rst[i].sum_loss, rel=1e-3)
+        assert sts[i].sum_dT == pytest.approx(rst[i].sum_dT, rel=2e-3, abs=0.05 * nx * ny * 1e-3)
+
+
 def test_stats_equal_host_reduction_of_field(ora):
     print("\n[case] f32 stats vs field", flush=True)
     nx, ny, n = 700, 300, 40
