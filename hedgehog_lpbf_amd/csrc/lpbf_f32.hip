@@ -77,6 +77,13 @@ __device__ __forceinline__ float from_right(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
 }
 
+// Empty register constraint: marks the value as produced at this point
+// (emits no instruction).
+template <typename T>
+__device__ __forceinline__ void pin(T& v) {
+    asm volatile("" : "+v"(v));
+}
+
 __device__ __forceinline__ float& elem(float4& v, int e) {
     return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w));
 }
@@ -113,9 +120,13 @@ __global__ __launch_bounds__(kThreads) void k_step_f32(F32Args a,
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         const LaserTick t = plan[step0 + j];
-        xc[j] = static_cast<float>(t.x_l * a.inv_dx);
-        yc[j] = static_cast<float>(t.y_l * a.inv_dy);
-        cS[j] = static_cast<float>(a.cS0 * t.q_scale);
+        // uniform values: keep them in SGPRs
+        xc[j] = __int_as_float(__builtin_amdgcn_readfirstlane(
+            __float_as_int(static_cast<float>(t.x_l * a.inv_dx))));
+        yc[j] = __int_as_float(__builtin_amdgcn_readfirstlane(
+            __float_as_int(static_cast<float>(t.y_l * a.inv_dy))));
+        cS[j] = __int_as_float(__builtin_amdgcn_readfirstlane(
+            __float_as_int(static_cast<float>(a.cS0 * t.q_scale))));
         srlo[j] = __builtin_amdgcn_readfirstlane(static_cast<int>(floorf(yc[j] - wcut)) - 1);
         srhi[j] = __builtin_amdgcn_readfirstlane(static_cast<int>(ceilf(yc[j] + wcut)) + 1);
         if (t.q_scale == 0.0) { srlo[j] = 1; srhi[j] = 0; }   // beam off: no source rows
@@ -201,36 +212,57 @@ __global__ __launch_bounds__(kThreads) void k_step_f32(F32Args a,
             return *reinterpret_cast<const float4*>(a.in + static_cast<long long>(ii) * a.ld + col4);
         };
 
-        // Three-slot ring per level: at ring phase ph the newest row sits in
-        // slot ph, the one before in (ph+2)%3 and the oldest in (ph+1)%3, so
-        // the window advances without register moves.  The general body
-        // always runs at phase 0 and rotates afterwards; the steady-state
-        // body runs the three phases back to back.
-        float4 W[K][3];
-        float4 PF[3];
+        // Register rings, indexed by a compile-time phase so the window
+        // advances without moves:
+        //   level 0: six slots holding rows i-2 .. i+3; at phase ph (mod 6)
+        //     row i (newest) is slot ph, i-1 is (ph+5)%6, i-2 is (ph+4)%6,
+        //     and row i+3 is loaded into slot (ph+3)%6, three rows ahead;
+        //   level j >= 1: three slots; at phase ph (mod 3) the newest row is
+        //     slot ph, the one before (ph+2)%3, the oldest (ph+1)%3.
+        // The steady-state loop runs six phases per trip (the common period),
+        // so every register keeps one role; the general body always runs at
+        // phase 0 and rotates afterwards.
+        float4 Z[6];
+        float4 W[K > 1 ? K - 1 : 1][3];   // W[j-1] is level j, j = 1..K-1
 #pragma unroll
-        for (int q = 0; q < 3; ++q) PF[q] = load_row(y_lo + q);
+        for (int q = 0; q < 3; ++q) Z[q] = load_row(y_lo + q);
+
+        auto win_new = [&](auto PHC, auto JC) -> float4& {
+            constexpr int ph = decltype(PHC)::value, j = decltype(JC)::value;
+            if constexpr (j == 0) return Z[ph % 6];
+            else return W[j - 1][ph % 3];
+        };
+        auto win_mid = [&](auto PHC, auto JC) -> float4& {
+            constexpr int ph = decltype(PHC)::value, j = decltype(JC)::value;
+            if constexpr (j == 0) return Z[(ph + 5) % 6];
+            else return W[j - 1][(ph + 2) % 3];
+        };
+        auto win_old = [&](auto PHC, auto JC) -> float4& {
+            constexpr int ph = decltype(PHC)::value, j = decltype(JC)::value;
+            if constexpr (j == 0) return Z[(ph + 4) % 6];
+            else return W[j - 1][(ph + 1) % 3];
+        };
 
         // One level-j row from level j-1's window.  GEN: any row of any strip
         // (edge columns, edge rows, band-halo rows, mirror rows); otherwise
         // an interior strip with every level on an owned interior row.
         auto level = [&](auto PHC, auto JC, auto GENC, int i) {
-            constexpr int ph = decltype(PHC)::value;
             constexpr int j = decltype(JC)::value;
             constexpr bool GEN = decltype(GENC)::value;
-            constexpr int s_new = ph, s_mid = (ph + 2) % 3, s_old = (ph + 1) % 3;
+            using JP = std::integral_constant<int, j - 1>;
             const int r = i - j;
             if (GEN) {
                 const int lo = max(ya - (K - j), 0);
                 const int hi = min(yb + (K - j), ny);
                 if (r < lo || r >= hi) {
-                    if (j < K && r == ny) W[j][s_new] = W[j][s_mid];   // row ny := row ny-1
+                    if constexpr (j < K)
+                        if (r == ny) win_new(PHC, JC) = win_mid(PHC, JC);   // row ny := row ny-1
                     return;
                 }
             }
-            const float4 c = W[j - 1][s_mid];
-            const float4 up = W[j - 1][s_old];
-            const float4 dn = W[j - 1][s_new];
+            const float4 c = win_mid(PHC, JP{});
+            const float4 up = win_old(PHC, JP{});
+            const float4 dn = win_new(PHC, JP{});
             const float wl = from_left(c.w);    // column left of c.x
             const float wr = from_right(c.x);   // column right of c.w
             const float cv[4] = {c.x, c.y, c.z, c.w};
@@ -308,45 +340,56 @@ __global__ __launch_bounds__(kThreads) void k_step_f32(F32Args a,
                 A.S += static_cast<double>(GEN ? row_sum(res) : own_sum(res));
                 A.SL += GEN ? row_sum(L4) : own_sum(L4);
                 A.SS += ssum;
+                // keep each running sum updated here: left alone the compiler
+                // defers the serial adds to the end of the unrolled trip and
+                // holds every phase's partial live (about 60 extra VGPRs)
+                pin(A.S);
+                pin(A.SL);
+                pin(A.SS);
             }
-            if (j < K) {
-                W[j][s_new] = res;
-                if (GEN && r == 0) W[j][s_mid] = res;   // row -1 := row 0
+            if constexpr (j < K) {
+                win_new(PHC, JC) = res;
+                if (GEN && r == 0) win_mid(PHC, JC) = res;   // row -1 := row 0
             }
         };
 
+        using J0 = std::integral_constant<int, 0>;
         auto body = [&](auto PHC, auto GENC, int i) {
             constexpr int ph = decltype(PHC)::value;
             constexpr bool GEN = decltype(GENC)::value;
-            constexpr int s_new = ph, s_mid = (ph + 2) % 3;
-            float4 v = PF[ph];
-            PF[ph] = GEN ? load_row(i + 3) : load_row_interior(i + 3);
+            Z[(ph + 3) % 6] = GEN ? load_row(i + 3) : load_row_interior(i + 3);
+            float4& v = win_new(PHC, J0{});
             if (!GEN || i < y_hi) {
                 if (GEN && BC == LPBF_BC_MIRROR) fix_mirror(v);
-                if (i >= ya && i < yb)
+                if (i >= ya && i < yb) {
                     S0 += static_cast<double>(GEN ? row_sum(v) : own_sum(v));
+                    pin(S0);
+                }
             } else if (i == ny) {
-                v = W[0][s_mid];                 // row ny := row ny-1
+                v = win_mid(PHC, J0{});                 // row ny := row ny-1
             }
-            W[0][s_new] = v;
-            if (GEN && i == 0) W[0][s_mid] = v;  // row -1 := row 0
+            if (GEN && i == 0) win_mid(PHC, J0{}) = v;  // row -1 := row 0
+            // levels run one after the other: the next level needs this
+            // level's row anyway, and interleaving them only raises pressure
             level(PHC, std::integral_constant<int, 1>{}, GENC, i);
-            if (K >= 2) level(PHC, std::integral_constant<int, (K >= 2 ? 2 : 1)>{}, GENC, i);
-            if (K >= 3) level(PHC, std::integral_constant<int, (K >= 3 ? 3 : 1)>{}, GENC, i);
-            if (K >= 4) level(PHC, std::integral_constant<int, (K >= 4 ? 4 : 1)>{}, GENC, i);
+            __builtin_amdgcn_sched_barrier(0);
+            if constexpr (K >= 2) level(PHC, std::integral_constant<int, 2>{}, GENC, i);
+            __builtin_amdgcn_sched_barrier(0);
+            if constexpr (K >= 3) level(PHC, std::integral_constant<int, 3>{}, GENC, i);
+            __builtin_amdgcn_sched_barrier(0);
+            if constexpr (K >= 4) level(PHC, std::integral_constant<int, 4>{}, GENC, i);
         };
-        using P0 = std::integral_constant<int, 0>;
-        using P1 = std::integral_constant<int, 1>;
-        using P2 = std::integral_constant<int, 2>;
         using Gen = std::true_type;
         using Fast = std::false_type;
         auto general = [&](int i) {
-            body(P0{}, Gen{}, i);
-            // back to the phase-0 layout: oldest <- previous, previous <- newest
+            body(std::integral_constant<int, 0>{}, Gen{}, i);
+            // back to the phase-0 layout
+            const float4 z0 = Z[0];
 #pragma unroll
-            for (int q = 0; q < K; ++q) { W[q][1] = W[q][2]; W[q][2] = W[q][0]; }
-            const float4 t = PF[0];
-            PF[0] = PF[1]; PF[1] = PF[2]; PF[2] = t;
+            for (int q = 0; q < 5; ++q) Z[q] = Z[q + 1];
+            Z[5] = z0;
+#pragma unroll
+            for (int q = 0; q + 1 < K; ++q) { W[q][1] = W[q][2]; W[q][2] = W[q][0]; }
         };
 
         // steady state: interior strip, every level on an owned row that is
@@ -356,10 +399,22 @@ __global__ __launch_bounds__(kThreads) void k_step_f32(F32Args a,
         const int i_e = interior ? min(yb + 1, ny - 1) : i_end + 1;
         int i = y_lo;
         for (; i <= i_end && i < i_s; ++i) general(i);
-        for (; i + 3 <= i_e; i += 3) {
-            body(P0{}, Fast{}, i);
-            body(P1{}, Fast{}, i + 1);
-            body(P2{}, Fast{}, i + 2);
+        // scheduling barriers keep the phases apart: interleaving them buys
+        // no latency the two-plus resident waves do not already hide, and
+        // costs registers
+        for (; i + 6 <= i_e; i += 6) {
+            body(std::integral_constant<int, 0>{}, Fast{}, i);
+            __builtin_amdgcn_sched_barrier(0);
+            body(std::integral_constant<int, 1>{}, Fast{}, i + 1);
+            __builtin_amdgcn_sched_barrier(0);
+            body(std::integral_constant<int, 2>{}, Fast{}, i + 2);
+            __builtin_amdgcn_sched_barrier(0);
+            body(std::integral_constant<int, 3>{}, Fast{}, i + 3);
+            __builtin_amdgcn_sched_barrier(0);
+            body(std::integral_constant<int, 4>{}, Fast{}, i + 4);
+            __builtin_amdgcn_sched_barrier(0);
+            body(std::integral_constant<int, 5>{}, Fast{}, i + 5);
+            __builtin_amdgcn_sched_barrier(0);
         }
         for (; i <= i_end; ++i) general(i);
     }
