@@ -101,14 +101,16 @@ def copy_reference(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=400)
     ap.add_argument("--workload", default="plate8k_f32",
                     choices=["plate8k_f32", "plate8k_f64", "copy"],
                     help="copy: a plain device copy of one n x n FP32 plate per step, the "
                          "reference for calibrating HBM byte counters")
-    ap.add_argument("--depth", type=int, default=4)
-    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--depth", type=int, default=0,
+                    help="FP32 steps per launch (1-4); 0 = pick depth and band rows with "
+                         "the library's autotuner before the warm-up")
+    ap.add_argument("--rows", type=int, default=0, help="FP32 band height (0 = built-in)")
     ap.add_argument("--n", type=int, default=8192, help="plate side (cells)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -135,11 +137,25 @@ def main():
     assert nsteps <= plan.shape[0]
     pdev = torch.from_numpy(plan).to(dev)
     stats = torch.empty(nsteps * 7 + 7, dtype=torch.float64, device=dev)
-    print(f"[bench] {args.workload} {nx}x{ny} warmup={args.warmup} steps={args.steps} "
-          f"depth={args.depth}", file=sys.stderr, flush=True)
+    print(f"[bench] {args.workload} {nx}x{ny} warmup={args.warmup} steps={args.steps}",
+          file=sys.stderr, flush=True)
 
+    tuned = None
     if args.workload == "plate8k_f32":
         depth, rows = args.depth, args.rows
+        if depth == 0:
+            bd, br, bms = C.c_int(), C.c_int(), C.c_double()
+            t0 = time.perf_counter()
+            rc = lib.lpbf_f32_autotune(k, 300.0, 0, 300.0, mat.Tm, pdev.data_ptr(), plan.shape[0],
+                                       4, st, C.byref(bd), C.byref(br), C.byref(bms))
+            if rc != 0:
+                h.check()
+                raise RuntimeError(f"autotune failed: {rc}")
+            depth, rows = bd.value, br.value
+            tuned = {"depth": depth, "rows": rows, "ms_per_step": bms.value,
+                     "tune_s": time.perf_counter() - t0}
+            print(f"[bench] autotune: depth {depth} rows {rows} ({bms.value:.4f} ms/step)",
+                  file=sys.stderr, flush=True)
         ld = (nx + 3) // 4 * 4
         a = torch.empty((ny, ld), dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
@@ -245,7 +261,9 @@ def main():
                         f"V 1.2 m/s, T0 300 K +/-0.5 K noise, "
                         f"{'FP32 theta field' if depth > 1 or args.workload == 'plate8k_f32' else 'FP64 field'}"
                         f", fused per-step stats",
-            "nx": nx, "ny": ny, "temporal_depth": depth, "band_rows": args.rows,
+            "nx": nx, "ny": ny, "temporal_depth": depth,
+            "band_rows": rows if args.workload == "plate8k_f32" else None,
+            "autotuned": tuned,
         },
         "roofline": {
             "bound": "hbm",

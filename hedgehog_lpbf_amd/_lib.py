@@ -86,6 +86,8 @@ SIGNATURES = {
     "lpbf_launch_step_f32": (_v, [_vp, _vp, _i, _KP, _d, _i, _d, _d, _vp, _i, _i, _i,
                                   _vp, _vp, _vp]),
     "lpbf_launch_f32_stats_finalize": (_v, [_vp, _i, _i, _i, _i, _vp, _vp]),
+    "lpbf_f32_autotune": (_i, [_KP, _d, _i, _d, _d, _vp, _i, _i, _vp, C.POINTER(C.c_int),
+                               C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "lpbf_launch_noise_f32": (_v, [_vp, _i, _i, _i, _ull, _d, _d, _vp]),
     "lpbf_launch_theta_to_T": (_v, [_vp, _i, _i, _i, _d, _vp, _vp]),
     "lpbf_launch_T_to_theta": (_v, [_vp, _i, _i, _d, _vp, _i, _vp]),

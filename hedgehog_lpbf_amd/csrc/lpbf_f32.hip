@@ -23,7 +23,6 @@
 //     consecutive tiles land on the same XCD (shared halo cache lines in L2).
 #include <hip/hip_runtime.h>
 
-#include <mutex>
 #include <type_traits>
 
 #include "lpbf_common.hpp"
@@ -503,39 +502,18 @@ __global__ void k_T_to_theta(const double* __restrict__ T, int nx, int ny, doubl
     }
 }
 
-// Band height: minimise the rows one SIMD streams, ceil(tiles/1024) *
-// (R + 2*depth), scaled by how well that many resident wavefronts per SIMD
-// hide latency (1: 0.6, 2: 0.85, >= 3: 1); the halo rows are recomputed
-// work, the tile count sets the tail and the occupancy.
-int rows_per_band_search(int nx, int ny, int depth) {
-    const int nstrips = (nx + kOutCols - 1) / kOutCols;
-    const int lo = depth > 1 ? 8 * depth : 16;
-    int best = lo;
-    double best_cost = 1e300;
-    for (int R = lo; R <= ny + lo; ++R) {
-        const long long tiles = static_cast<long long>(nstrips) * ((ny + R - 1) / R);
-        const long long per = (tiles + 1023) / 1024;
-        const double eff = per >= 3 ? 1.0 : (per == 2 ? 0.85 : 0.6);
-        const double cost = static_cast<double>(per) * (R + 2 * depth) / eff;
-        if (cost < best_cost - 1e-9) { best_cost = cost; best = R; }
-        if (R >= ny) break;
-    }
-    return best;
-}
-
+// Built-in band height: about six resident wavefront tiles per SIMD (1024
+// SIMDs) so latency is hidden, but never below 8*depth rows, where the
+// 2*depth recomputed halo rows would dominate.  lpbf_f32_autotune measures
+// instead of guessing.
 int rows_per_band(int nx, int ny, int depth) {
-    // the search is O(ny); remember the last few shapes
-    struct Entry { int nx, ny, depth, R; };
-    static std::mutex mu;
-    static Entry cache[16];
-    static int next = 0;
-    std::lock_guard<std::mutex> g(mu);
-    for (const Entry& e : cache)
-        if (e.nx == nx && e.ny == ny && e.depth == depth && e.R > 0) return e.R;
-    const int R = rows_per_band_search(nx, ny, depth);
-    cache[next] = {nx, ny, depth, R};
-    next = (next + 1) % 16;
-    return R;
+    const long long nstrips = (nx + kOutCols - 1) / kOutCols;
+    const long long bands = (6 * 1024 + nstrips - 1) / nstrips;
+    int R = static_cast<int>((ny + bands - 1) / bands);
+    const int lo = depth > 1 ? 8 * depth : 16;
+    if (R < lo) R = lo;
+    if (R > ny) R = ny;
+    return R < 1 ? 1 : R;
 }
 
 struct Geometry {
@@ -669,6 +647,74 @@ void lpbf_launch_f32_stats_finalize(const void* d_work, int nx, int ny, int dept
     hipLaunchKernelGGL(k_stats_finalize_levels, dim3(depth), dim3(kThreads), 0, stream,
                        static_cast<const StepStats*>(d_work), g.nwg, nx, ny, d_stats);
     check_launch();
+}
+
+int lpbf_f32_autotune(const StepConst* k, double T_ref, int bc, double T_bc, double Tm,
+                      const LaserTick* d_plan, int n_plan, int max_depth, hipStream_t stream,
+                      int* best_depth, int* best_rows, double* best_ms_per_step) {
+    if (!k || !f32_shape_ok(k->nx, k->ny) || !d_plan || max_depth < 1 || max_depth > kMaxDepth ||
+        n_plan < max_depth || !best_depth || !best_rows ||
+        (bc != LPBF_BC_MIRROR && bc != LPBF_BC_DIRICHLET)) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return LPBF_ERR_INVALID_ARGUMENT;
+    }
+    const int nx = k->nx, ny = k->ny, ld = (nx + 3) & ~3;
+    static const int kRows[] = {0, 16, 24, 32, 40, 46, 56, 64, 92, 128, 192};
+    size_t work_bytes = 0;
+    for (int d = 1; d <= max_depth; ++d)
+        for (int R : kRows) {
+            if (R > ny) continue;
+            const size_t b = lpbf_f32_workspace_bytes(nx, ny, d, R);
+            if (b > work_bytes) work_bytes = b;
+        }
+    float* a = nullptr;
+    float* b = nullptr;
+    void* work = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const size_t field = sizeof(float) * static_cast<size_t>(ld) * ny;
+    hipError_t err = hipMalloc(reinterpret_cast<void**>(&a), field);
+    if (err == hipSuccess) err = hipMalloc(reinterpret_cast<void**>(&b), field);
+    if (err == hipSuccess) err = hipMalloc(&work, work_bytes);
+    if (err == hipSuccess) err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    double best = 1e300;
+    int bd = 1, br = 0;
+    if (err == hipSuccess) {
+        lpbf_launch_noise_f32(a, nx, ny, ld, 1, 0.0, 0.5, stream);
+        const int reps = 3;
+        const int d_first = (max_depth == 1) ? 1 : 2;
+        for (int depth = d_first; depth <= max_depth && err == hipSuccess; ++depth) {
+            for (int R : kRows) {
+                if (R > ny || (R > 0 && R < 2 * depth)) continue;
+                lpbf_launch_step_f32(a, b, ld, k, T_ref, bc, T_bc, Tm, d_plan, 0, depth, R, nullptr,
+                                     work, stream);   // warm-up
+                hipEventRecord(e0, stream);
+                for (int q = 0; q < reps; ++q)
+                    lpbf_launch_step_f32((q & 1) ? a : b, (q & 1) ? b : a, ld, k, T_ref, bc, T_bc,
+                                         Tm, d_plan, 0, depth, R, nullptr, work, stream);
+                hipEventRecord(e1, stream);
+                err = hipEventSynchronize(e1);
+                if (err != hipSuccess) break;
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, e0, e1);
+                const double per_step = ms / (reps * depth);
+                if (per_step < best) { best = per_step; bd = depth; br = R; }
+            }
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    if (work) hipFree(work);
+    if (err != hipSuccess) {
+        record_error(static_cast<int>(err));
+        return static_cast<int>(err);
+    }
+    *best_depth = bd;
+    *best_rows = br;
+    if (best_ms_per_step) *best_ms_per_step = best;
+    return 0;
 }
 
 void lpbf_launch_noise_f32(float* theta, int nx, int ny, int ld, unsigned long long seed,

@@ -128,6 +128,16 @@ void   lpbf_launch_f32_stats_finalize(const void* d_work, int nx, int ny,
                                       int depth, int rows, StepStats* d_stats,
                                       hipStream_t stream);
 
+/* Autotuner: times lpbf_launch_step_f32 on scratch copies of a k->nx x
+ * k->ny field (allocated and freed here, synchronising `stream`) for
+ * temporal depths 2..max_depth (1 if max_depth == 1) and a set of band
+ * heights, and returns the fastest per step.  d_plan needs max_depth entries.
+ * Returns 0 or an error code (also recorded for lpbf_get_last_error). */
+int    lpbf_f32_autotune(const StepConst* k, double T_ref, int bc, double T_bc,
+                         double Tm, const LaserTick* d_plan, int n_plan,
+                         int max_depth, hipStream_t stream, int* best_depth,
+                         int* best_rows, double* best_ms_per_step);
+
 /* theta = base + 2*amp*u(seed, y*nx+x) (same u as lpbf_launch_noise_f64). */
 void   lpbf_launch_noise_f32(float* theta, int nx, int ny, int ld,
                              unsigned long long seed, double base, double amp,
