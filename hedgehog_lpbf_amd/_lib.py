@@ -54,7 +54,21 @@ class StepStats(C.Structure):
                 ("sum_src", C.c_double), ("sum_loss", C.c_double)]
 
 
+class RunOptions(C.Structure):
+    """include/lpbf_run.h lpbf_run_options."""
+    _fields_ = [("precision", C.c_int), ("bc", C.c_int), ("T_bc", C.c_double),
+                ("batch", C.c_int), ("depth", C.c_int), ("rows", C.c_int),
+                ("T_ref", C.c_double), ("keep_records", C.c_int)]
+
+
+class StepRecord(C.Structure):
+    """include/lpbf_run.h lpbf_step_record."""
+    _fields_ = [("stats", StepStats), ("W_um", C.c_double), ("W_sub_um", C.c_double),
+                ("step", C.c_int), ("has_col", C.c_int)]
+
+
 assert C.sizeof(StepConst) == 216 and C.sizeof(Partial) == 24
+assert C.sizeof(StepRecord) == 80
 assert C.sizeof(LaserTick) == 24 and C.sizeof(StepStats) == 56
 
 _vp, _i, _d, _v = C.c_void_p, C.c_int, C.c_double, None
@@ -91,6 +105,17 @@ SIGNATURES = {
     "lpbf_launch_noise_f32": (_v, [_vp, _i, _i, _i, _ull, _d, _d, _vp]),
     "lpbf_launch_theta_to_T": (_v, [_vp, _i, _i, _i, _d, _vp, _vp]),
     "lpbf_launch_T_to_theta": (_v, [_vp, _i, _i, _d, _vp, _i, _vp]),
+    "lpbf_launch_extract_col_f32": (_v, [_vp, _i, _i, _i, _d, _vp, _vp, _vp]),
+    # device-resident run (include/lpbf_run.h)
+    "lpbf_run_create": (_i, [_KP, _i, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _i,
+                             C.POINTER(RunOptions), C.POINTER(_vp)]),
+    "lpbf_run_destroy": (_v, [_vp]),
+    "lpbf_run_upload": (_i, [_vp, _vp]),
+    "lpbf_run_noise": (_i, [_vp, _ull, _d, _d]),
+    "lpbf_run_advance": (_i, [_vp, _i]),
+    "lpbf_run_download": (_i, [_vp, _vp]),
+    "lpbf_run_summary": (_i, [_vp, _DP, C.POINTER(C.c_int)]),
+    "lpbf_run_records": (_i, [_vp, _i, _i, C.POINTER(StepRecord)]),
     # host helpers (include/lpbf_host.h)
     "lpbf_make_const": (_v, [_DP, _DP, _i, _d, _d, _i, _i, _d, _KP]),
     "lpbf_dt_cfl": (_d, [_DP, _d, _d, _i, _i]),

@@ -492,6 +492,16 @@ __global__ void k_theta_to_T(const float* __restrict__ th, int ld, int nx, int n
     }
 }
 
+// Column x_hot = glob->arg % nx of T = T_ref + theta, as FP64.
+__global__ void k_extract_col_f32(const float* __restrict__ th, int ld, int nx, int ny,
+                                  double Tref, const StepStats* __restrict__ glob,
+                                  double* __restrict__ col) {
+    const int y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= ny) return;
+    const int x_hot = static_cast<int>(glob->arg % nx);
+    col[y] = Tref + static_cast<double>(th[static_cast<long long>(y) * ld + x_hot]);
+}
+
 __global__ void k_T_to_theta(const double* __restrict__ T, int nx, int ny, double Tref,
                              float* __restrict__ th, int ld) {
     const long long n = static_cast<long long>(nx) * ny;
@@ -742,6 +752,18 @@ void lpbf_launch_theta_to_T(const float* theta, int ld, int nx, int ny, double T
     const int grid = static_cast<int>(want < 16384 ? want : 16384);
     hipLaunchKernelGGL(k_theta_to_T, dim3(grid), dim3(kThreads), 0, stream, theta, ld, nx, ny, T_ref,
                        T);
+    check_launch();
+}
+
+void lpbf_launch_extract_col_f32(const float* theta, int ld, int nx, int ny, double T_ref,
+                                 const StepStats* glob, double* out_col, hipStream_t stream) {
+    if (!theta || !glob || !out_col || !f32_shape_ok(nx, ny) || ld < nx) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    const int bs = 128;
+    hipLaunchKernelGGL(k_extract_col_f32, dim3((ny + bs - 1) / bs), dim3(bs), 0, stream, theta, ld,
+                       nx, ny, T_ref, glob, out_col);
     check_launch();
 }
 

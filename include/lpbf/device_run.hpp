@@ -1,22 +1,26 @@
 // SPDX-License-Identifier: 0BSD
-// Device-resident run orchestration: the reference GpuStepTask batched path
-// (include/gpu_step_task.hpp:135-239) and MetricsTask finalisation
-// (:375-411), without Hedgehog.
+// Device-resident run orchestration: what the reference GpuStepTask batched
+// path (include/gpu_step_task.hpp:135-239, mode 2, fused and pinned) and
+// MetricsTask (:375-411) do together, without Hedgehog.
 //
-// The field stays in HBM (double buffer).  Each batch of up to `batch` steps
-// is launched back to back on one stream: the step with fused statistics,
-// the per-step merge into a device ring, and on cruise steps the hot-column
-// copy; then one device-to-host copy of the rings and one stream
-// synchronisation, after which the per-step metrics (W, W_sub, running
-// maxima, the cruise-gated steady W_sub) are finalised in step order.
+// The field stays in HBM (double buffer).  A batch of up to `batch` steps is
+// launched back to back on the run's own stream: the step with fused
+// statistics, the per-step merge into a device ring, on cruise steps the
+// hot-column copy into a column ring, and on dump steps an asynchronous copy
+// of the field to host memory; then one device-to-host copy of the rings
+// (pinned staging) and one stream synchronisation, after which the per-step
+// metrics (W, W_sub, running maxima, the cruise-gated steady W_sub) are
+// finalised on the host in step order, exactly as MetricsTask does.
 //
-// Precision 64 runs the FP64 parity kernels (one step per launch);
-// precision 32 runs the FP32 path with `depth` steps per launch, so the hot
-// column, and with it W_sub, exists only for the last step of each launch.
+// Precision 64 runs the FP64 parity kernels, one step per launch, so every
+// per-step value equals the reference's.  Precision 32 runs the temporally
+// blocked FP32 kernel with up to `depth` steps per launch; a launch never
+// crosses a dump step or a batch end, and the hot column (hence W_sub) is
+// taken at the last step of each launch only.
 #pragma once
 
 #include <cstdint>
-#include <string>
+#include <functional>
 #include <vector>
 
 #include "../lpbf_step_const.h"
@@ -28,66 +32,88 @@ namespace lpbf {
 struct RunOptions {
     int precision = 64;            // 64 or 32
     int bc = LPBF_BC_MIRROR;
-    double T_bc = 300.0;
-    Physics phys = Physics::Baseline;   // extended physics: precision 64 only
-    double kmult = 1.0;
+    double T_bc = 300.0;           // Dirichlet edge temperature
     int batch = 64;                // steps per host synchronisation
     int depth = 4;                 // FP32 steps per launch (1..4)
     int rows = 0;                  // FP32 band height (0 = built-in)
     double T_ref = -1.0;           // FP32 theta origin (< 0: Tamb)
+    bool keep_records = true;      // keep one StepRecord per step
 };
 
 struct StepRecord {
-    Tick tick;
-    StepStats stats;
+    int step = 0;
+    bool has_col = false;          // W_sub was computed from a hot column
+    StepStats stats{};
     double W_um = 0.0;
     double W_sub_um = 0.0;
-    bool has_col = false;
 };
 
 class DeviceRun {
 public:
-    DeviceRun(Material mat, Process pr, Domain dom, double dt, std::vector<Tick> plan,
-              RunOptions opt);
+    // dom/k/phys/Tm/plan/Tamb have the meaning of the reference GpuStepTask
+    // constructor arguments (gpu_step_task.hpp:63-66); k->kmult is honoured.
+    // Throws std::invalid_argument / std::runtime_error on bad input or a
+    // HIP failure.
+    DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<Tick> plan, double Tamb,
+              RunOptions opt = {});
     ~DeviceRun();
     DeviceRun(DeviceRun const&) = delete;
     DeviceRun& operator=(DeviceRun const&) = delete;
 
-    void upload(const double* T);          // nx*ny row-major FP64
-    void noise(std::uint64_t seed, double base, double amp);
-    int advance(int n_steps);              // returns steps actually run
-    void download(double* T) const;        // current field as FP64 T
+    // Called for every dump step (plan[s].dump) in step order, after the batch
+    // that contains it; `field` is the FP64 row-major T after step s.
+    void on_snapshot(std::function<void(int step, std::vector<double>&& field)> cb) {
+        snap_cb_ = std::move(cb);
+    }
+
+    void upload(const double* T);                                 // nx*ny FP64
+    void noise(std::uint64_t seed, double base, double amp);      // T = base + U(-amp, amp)
+    int advance(int n_steps);      // runs up to n_steps more steps; returns how many
+    int run() { return advance(static_cast<int>(plan_.size()) - done_); }
+    void download(double* T) const;                               // current T, FP64
 
     int steps_done() const noexcept { return done_; }
+    int n_steps() const noexcept { return static_cast<int>(plan_.size()); }
     double peak_T() const noexcept { return peak_T_; }
     double peak_W() const noexcept { return peak_W_; }
     double steady_W_sub() const noexcept { return steady_; }
     std::vector<StepRecord> const& records() const noexcept { return rec_; }
+    StepRecord const& last() const noexcept { return last_; }
     void* stream() const noexcept { return stream_; }
 
 private:
     void batch(int bn);
+    void release() noexcept;
+    bool cruise(Tick const& t) const noexcept {
+        return !t.wrapped && t.x_laser_csv >= 0.45 * dom_.Lx && t.x_laser_csv <= 0.85 * dom_.Lx;
+    }
+    void finish_step(Tick const& t, StepStats const& st, const double* col);
 
-    Material mat_;
-    Process pr_;
     Domain dom_;
-    double dt_;
+    StepConst k_;
+    int phys_;
+    double Tm_;
     std::vector<Tick> plan_;
+    double Tamb_;
     RunOptions opt_;
-    StepConst k_{};
+    std::function<void(int, std::vector<double>&&)> snap_cb_;
+
     int nx_, ny_, ld_ = 0, nblk_ = 0;
+    double T_ref_ = 0.0;
     void* stream_ = nullptr;
-    void* d_a_ = nullptr;          // current field
+    void* d_a_ = nullptr;          // current field (FP64 T or FP32 theta)
     void* d_b_ = nullptr;
-    void* d_plan_ = nullptr;       // LaserTick[n_steps]
-    void* d_blk_ = nullptr;        // block records / FP32 workspace
-    StepStats* d_ring_ = nullptr;  // per-step stats of the batch
-    double* d_cols_ = nullptr;     // per-step hot columns of the batch
-    std::vector<StepStats> h_ring_;
-    std::vector<double> h_cols_;
+    LaserTick* d_plan_ = nullptr;  // FP32 path: laser per step
+    void* d_blk_ = nullptr;        // FP64 block records / FP32 workspace
+    StepStats* d_ring_ = nullptr;  // per-step stats of a batch
+    double* d_cols_ = nullptr;     // per-step hot columns of a batch
+    double* d_snap_ = nullptr;     // FP32 path: FP64 staging for snapshots
+    StepStats* h_ring_ = nullptr;  // pinned
+    double* h_cols_ = nullptr;     // pinned
     std::vector<StepRecord> rec_;
+    StepRecord last_;
     int done_ = 0;
-    double peak_T_, peak_W_ = 0.0, steady_ = 0.0;
+    double peak_T_ = 0.0, peak_W_ = 0.0, steady_ = 0.0;
 };
 
 }  // namespace lpbf

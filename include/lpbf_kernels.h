@@ -143,6 +143,13 @@ void   lpbf_launch_noise_f32(float* theta, int nx, int ny, int ld,
                              unsigned long long seed, double base, double amp,
                              hipStream_t stream);
 
+/* FP32 counterpart of lpbf_launch_extract_col: out_col[y] = T_ref +
+ * theta[y*ld + x_hot] with x_hot = glob->arg % nx (glob: the step's merged
+ * StepStats, whose first fields are a Partial). */
+void   lpbf_launch_extract_col_f32(const float* theta, int ld, int nx, int ny,
+                                   double T_ref, const StepStats* glob,
+                                   double* out_col, hipStream_t stream);
+
 /* Conversions between the FP64 row-major T and the pitched FP32 theta. */
 void   lpbf_launch_theta_to_T(const float* theta, int ld, int nx, int ny,
                               double T_ref, double* T, hipStream_t stream);

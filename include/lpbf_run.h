@@ -1,0 +1,77 @@
+/* SPDX-License-Identifier: 0BSD
+ *
+ * C ABI of the device-resident run (lpbf::DeviceRun, include/lpbf/
+ * device_run.hpp): the reference GpuStepTask + MetricsTask pair
+ * (include/gpu_step_task.hpp:135-239, 375-411) behind an opaque handle, for
+ * callers that are not C++ (ctypes, cgo, JNI).
+ *
+ * Every call returns 0 on success or an error code (a hipError_t value or
+ * LPBF_ERR_INVALID_ARGUMENT), also recorded for lpbf_get_last_error().
+ * lpbf_run_advance returns the number of steps run, or a negative code.
+ * Calls on one handle must not overlap (one host thread owns a run).
+ */
+#ifndef LPBF_RUN_H
+#define LPBF_RUN_H
+
+#include "lpbf_step_const.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lpbf_run_options {
+    int    precision;     /* 64 (FP64 parity kernels) or 32 (FP32 path) */
+    int    bc;            /* LPBF_BC_MIRROR or LPBF_BC_DIRICHLET */
+    double T_bc;          /* Dirichlet edge temperature [K] */
+    int    batch;         /* steps per host synchronisation, >= 1 */
+    int    depth;         /* FP32: steps per launch, 1..4 */
+    int    rows;          /* FP32: band height, 0 = built-in */
+    double T_ref;         /* FP32: theta origin [K], < 0 = Tamb */
+    int    keep_records;  /* 1: keep one lpbf_step_record per step */
+} lpbf_run_options;
+
+/* Per-step result: the merged statistics of the field after the step, the
+ * melt-pool width from the molten rows and the sub-grid width through the
+ * hottest column (0 when no column was taken). */
+typedef struct lpbf_step_record {
+    StepStats stats;
+    double    W_um;
+    double    W_sub_um;
+    int       step;
+    int       has_col;
+} lpbf_step_record;
+
+typedef struct lpbf_run lpbf_run;
+
+/* Laser plan of n_steps ticks in the reference Tick sense
+ * (include/gpu_messages.hpp:18-27): plan[s] is the source during step s;
+ * x_csv[s] / wrapped[s] the post-advance position and wrap state (cruise
+ * gate); dump[s] marks snapshot steps.  x_csv, wrapped, dump may be NULL
+ * (no cruise steps, no dumps).  k holds the grid, physics and kmult;
+ * Lx, Ly the plate size (cruise gate, W = rows * Ly/(ny-1)); phys is
+ * LPBF_PHYS_*; the field starts uniform at Tamb. */
+int  lpbf_run_create(const StepConst* k, int phys, double Tm, double Lx, double Ly, double Tamb,
+                     const LaserTick* plan, const double* x_csv,
+                     const unsigned char* wrapped, const unsigned char* dump,
+                     int n_steps, const lpbf_run_options* opt, lpbf_run** out);
+void lpbf_run_destroy(lpbf_run* run);
+
+int  lpbf_run_upload(lpbf_run* run, const double* T);            /* nx*ny FP64 */
+int  lpbf_run_noise(lpbf_run* run, unsigned long long seed, double base, double amp);
+int  lpbf_run_advance(lpbf_run* run, int n_steps);
+int  lpbf_run_download(lpbf_run* run, double* T);                /* nx*ny FP64 */
+
+/* out[0..2] = peak T [K], peak W [um], steady W_sub [um]; *steps_done. */
+int  lpbf_run_summary(const lpbf_run* run, double* out3, int* steps_done);
+
+/* Copies records [first, first+n) (requires keep_records). */
+int  lpbf_run_records(const lpbf_run* run, int first, int n, lpbf_step_record* out);
+
+#ifdef __cplusplus
+}
+static_assert(sizeof(lpbf_step_record) == 80, "lpbf_step_record layout");
+#else
+_Static_assert(sizeof(lpbf_step_record) == 80, "lpbf_step_record layout");
+#endif
+
+#endif /* LPBF_RUN_H */

@@ -56,6 +56,17 @@ def test_invalid_arguments_are_recorded():
         h.check()
 
 
+def test_run_create_rejects_bad_options_before_touching_a_device():
+    k = h.StepConst()
+    k.nx, k.ny, k.dt = 64, 32, 1e-6
+    plan = np.zeros((4, 3))
+    for kw in (dict(precision=16), dict(bc=5), dict(precision=32, depth=9),
+               dict(precision=32, phys=1), dict(rows=-1)):
+        with pytest.raises(h.LpbfError):
+            h.DeviceRun(k, Tm=1609.0, Lx=1e-3, Ly=1e-3, Tamb=300.0, plan=plan, **kw)
+    assert h.lib().lpbf_get_last_error() == 0
+
+
 def test_make_const_and_dt_equal_oracle(ora):
     for case in ("0", "1.1", "3.2"):
         s = ora.defaults(case)

@@ -1,0 +1,192 @@
+"""Device-resident orchestration (lpbf::DeviceRun through include/lpbf_run.h,
+and the lpbf_gpu driver): per-step records, running maxima, snapshots and
+output formats against the CPU oracle and the reference case0 files.
+
+Tolerances: FP64 fields and peak temperatures 1e-9 K; melt widths from
+molten rows exact; sub-grid widths (linear interpolation of the Tm crossing,
+which divides by a temperature difference) 1e-6 um.  FP32: peak temperature
+within 0.5 K of the FP64 oracle."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import hedgehog_lpbf_amd as h
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+EXE = os.path.join(ROOT, "hedgehog_lpbf_amd", "lib", "lpbf_gpu")
+
+
+def _case0():
+    mat, pr, dom = h.Material(), h.apply_case("0", h.Process()), h.Domain()
+    dt = h.dt_cfl(mat, dom)
+    n = int(1.5e-3 / dt)
+    k = h.make_const(mat, pr, dom, dt)
+    plan, x_csv, wrapped = h.single_track_plan(pr.V, dt, dom.Lx, dom.Ly, n)
+    return mat, pr, dom, dt, n, k, plan, x_csv, wrapped
+
+
+def _run(k, mat, pr, dom, plan, x_csv, wrapped, **kw):
+    return h.DeviceRun(k, Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, Tamb=pr.Tamb, plan=plan, x_csv=x_csv,
+                       wrapped=wrapped, **kw)
+
+
+def test_case0_device_run_fp64_matches_oracle(ora):
+    print("\n[case] DeviceRun case0 FP64 batch 64", flush=True)
+    mat, pr, dom, dt, n, k, plan, x_csv, wrapped = _case0()
+    with _run(k, mat, pr, dom, plan, x_csv, wrapped, batch=64) as run:
+        assert run.advance(n) == n
+        recs = run.records()
+        peak_T, peak_W, steady, done = run.summary()
+        T = run.download()
+    s = ora.defaults("0")
+    T_o, pk, W, Ws, summ = ora.run(s, ora.dt_cfl(s), n)
+    assert done == n and len(recs) == n
+    assert np.abs(T - T_o).max() <= 1e-9
+    tmax = np.array([r.stats.tmax for r in recs])
+    assert np.abs(tmax - pk).max() <= 1e-9
+    assert np.array_equal(np.array([r.W_um for r in recs]), W)
+    cruise = (~wrapped) & (x_csv >= 0.45 * dom.Lx) & (x_csv <= 0.85 * dom.Lx)
+    has = np.array([bool(r.has_col) for r in recs])
+    assert np.array_equal(has, cruise)
+    Wsub = np.array([r.W_sub_um for r in recs])
+    assert cruise.sum() > 100
+    assert np.abs(Wsub[cruise] - Ws[cruise]).max() <= 1e-6
+    assert abs(peak_T - summ[0]) <= 1e-9 and peak_W == summ[1]
+    assert abs(steady - summ[2]) <= 1e-6
+    gold = json.load(open(os.path.join(GOLD, "case0_meta.json")))
+    assert "%g" % peak_T == "%g" % gold["peak_T_K"]
+    assert "%g" % peak_W == "%g" % gold["peak_W_um"]
+
+
+@pytest.mark.parametrize("batch", [1, 7, 300])
+def test_batch_size_does_not_change_results(batch):
+    print(f"\n[case] DeviceRun case0 FP64 200 steps batch {batch}", flush=True)
+    mat, pr, dom, dt, n, k, plan, x_csv, wrapped = _case0()
+    n = 200
+    plan, x_csv, wrapped = plan[:n], x_csv[:n], wrapped[:n]
+    # shift the cruise window so it starts inside this short run
+    x_csv = x_csv + 0.3 * dom.Lx
+    with _run(k, mat, pr, dom, plan, x_csv, wrapped, batch=64) as a:
+        a.advance(n)
+        ra, Ta = a.records(), a.download()
+    with _run(k, mat, pr, dom, plan, x_csv, wrapped, batch=batch) as b:
+        got = 0
+        while got < n:            # uneven advance calls across batch boundaries
+            got += b.advance(min(37, n - got))
+        rb, Tb = b.records(), b.download()
+    assert np.array_equal(Ta, Tb)
+    assert [bytes(r) for r in ra] == [bytes(r) for r in rb]
+
+
+def test_dirichlet_noise_run_matches_oracle(ora):
+    print("\n[case] DeviceRun 160x96 Dirichlet + noise FP64 300 steps", flush=True)
+    mat, pr, dom = h.Material(k=7.0, rho=4430.0, cp=526.0), h.Process(P=200.0, V=1.0, r0=50e-6,
+                                                                      h=50e-6), h.Domain()
+    pr.Tamb = pr.Tsub = 300.0
+    dom.nx, dom.ny, dom.Lx, dom.Ly = 160, 96, 159e-5, 95e-5
+    dt = 5e-6
+    k = h.make_const(mat, pr, dom, dt)
+    n = 300
+    plan, x_csv, wrapped = h.single_track_plan(pr.V, dt, dom.Lx, dom.Ly, n)
+    with _run(k, mat, pr, dom, plan, x_csv, wrapped, bc=h.LPBF_BC_DIRICHLET, T_bc=300.0,
+              batch=50) as run:
+        run.noise(0, 300.0, 0.5)
+        T0 = run.download()
+        run.advance(n)
+        T = run.download()
+        recs = run.records()
+    assert np.array_equal(T0, ora.noise_field(0, 300.0, 0.5, (dom.ny, dom.nx)))
+    T_o, st = ora.run_plan(k, plan, T0, 0, h.LPBF_BC_DIRICHLET, 300.0, mat.Tm)
+    assert np.abs(T - T_o).max() <= 1e-9
+    for r, o in zip(recs, st):
+        assert abs(r.stats.tmax - o.tmax) <= 1e-9 and r.stats.arg == o.arg
+        assert (r.stats.ymin, r.stats.ymax, r.stats.molten) == (o.ymin, o.ymax, o.molten)
+        assert abs(r.stats.sum_dT - o.sum_dT) <= 1e-9 * dom.nx * dom.ny
+
+
+def test_fp32_device_run_within_half_kelvin(ora):
+    print("\n[case] DeviceRun case0 FP32 depth 4 batch 64", flush=True)
+    mat, pr, dom, dt, n, k, plan, x_csv, wrapped = _case0()
+    n = 600
+    plan, x_csv, wrapped = plan[:n], x_csv[:n], wrapped[:n]
+    dump = np.zeros(n, dtype=bool)
+    dump[[0, 101, 333, n - 1]] = True
+    with h.DeviceRun(k, Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, Tamb=pr.Tamb, plan=plan, x_csv=x_csv,
+                     wrapped=wrapped, dump=dump, precision=32, depth=4, batch=64) as run:
+        run.advance(n)
+        recs = run.records()
+        T = run.download()
+        peak_T = run.summary()[0]
+    T_o, st = ora.run_plan(k, plan, np.full((dom.ny, dom.nx), pr.Tamb))
+    assert np.abs(T - T_o).max() <= 0.5
+    tmax = np.array([r.stats.tmax for r in recs])
+    assert np.abs(tmax - np.array([o.tmax for o in st])).max() <= 0.5
+    assert abs(peak_T - max(o.tmax for o in st)) <= 0.5
+    # hot columns only at launch ends: launches of 4 steps that stop at dump
+    # steps and batch ends
+    cruise = (~wrapped) & (x_csv >= 0.45 * dom.Lx) & (x_csv <= 0.85 * dom.Lx)
+    ends = set()
+    s = 0
+    while s < n:
+        b_end = (s // 64 + 1) * 64
+        L = 1
+        while L < 4 and s + L < min(b_end, n) and not dump[s + L - 1]:
+            L += 1
+        ends.add(s + L - 1)
+        s += L
+    has = np.array([bool(r.has_col) for r in recs])
+    want = np.array([(i in ends) and cruise[i] for i in range(n)])
+    assert np.array_equal(has, want)
+
+
+def test_lpbf_gpu_driver_writes_reference_formats(tmp_path, ora):
+    print("\n[case] lpbf_gpu case0 snap-every 200", flush=True)
+    out = tmp_path / "case0"
+    r = subprocess.run([EXE, "--case=0", "--snap-every=200", "--threads=3", f"--out={out}"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert (out / "metrics.csv").read_text() == open(
+        os.path.join(GOLD, "case0_metrics.csv")).read()
+    meta = json.load(open(out / "meta.json"))
+    gold = json.load(open(os.path.join(GOLD, "case0_meta.json")))
+    for key in ("dt_s", "n_steps", "peak_T_K", "peak_W_um", "nx", "ny", "P_W", "V_mps", "r0_m"):
+        assert "%g" % meta[key] == "%g" % gold[key], key
+    snaps = sorted(p.name for p in out.glob("T_step_*.bin"))
+    assert snaps == ["T_step_%06d.bin" % s for s in (0, 200, 400, 600, 800, 1000, 1200, 1400,
+                                                     1426)]
+    s = ora.defaults("0")
+    T_o = ora.run(s, ora.dt_cfl(s), 1427)[0]
+    T = np.fromfile(out / "T_step_001426.bin", dtype=np.float64).reshape(120, 240)
+    assert np.abs(T - T_o).max() <= 1e-9
+
+
+def test_lpbf_gpu_manifest_and_quiet(tmp_path, ora):
+    print("\n[case] lpbf_gpu --manifest (2 rows) and --quiet", flush=True)
+    man = tmp_path / "runs.csv"
+    man.write_text("run_id,cond_id,draw_id,config,P_W,V_mps,r0_m,eta,dsub_m,kmult,physics\n"
+                   "a,c0,0,single,285,0.96,3.35e-05,0.35,3e-05,1,base\n"
+                   "b,c1,1,single,325,1.2,3.35e-05,0.4,3e-05,1,base\n")
+    r = subprocess.run([EXE, "--manifest=%s" % man, "--tend=4e-4"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rows = [ln.split(",") for ln in r.stdout.splitlines() if ln.startswith("RESULT,")]
+    assert [x[1] for x in rows] == ["a", "b"]
+    for row, (P, V, eta) in zip(rows, [(285.0, 0.96, 0.35), (325.0, 1.2, 0.4)]):
+        s = ora.defaults("0")
+        s.proc.P, s.proc.V, s.mat.eta = P, V, eta
+        dt = ora.dt_cfl(s)
+        n = int(4e-4 / dt)
+        summ = ora.run(s, dt, n)[4]
+        assert int(row[15]) == n
+        assert row[13] == "%.6g" % summ[0]          # peak_T_K
+        assert row[12] == "%.6g" % summ[2]          # steady_W_sub_um
+    q = subprocess.run([EXE, "--case=0", "--quiet", "--tend=4e-4"], capture_output=True,
+                       text=True, timeout=300)
+    assert q.returncode == 0, q.stderr
+    qrow = [ln for ln in q.stdout.splitlines() if ln.startswith("RESULT,")][0].split(",")
+    assert qrow[12:14] == rows[0][12:14]
