@@ -67,6 +67,12 @@ void StepTileTask::reserve(std::size_t n) {
 }
 
 void StepTileTask::execute(std::shared_ptr<HaloTile> tile) {
+    step(*tile);
+    this->addResult(tile);
+}
+
+void StepTileTask::step(HaloTile& tile_ref) {
+    HaloTile* tile = &tile_ref;
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     const int nx = tile->nx;
@@ -97,7 +103,6 @@ void StepTileTask::execute(std::shared_ptr<HaloTile> tile) {
     const auto ts1 = clk::now();
 
     tile->data = std::move(T_new);
-    this->addResult(tile);
 
     const auto t1 = clk::now();
     if (prof_) {

@@ -38,6 +38,9 @@ public:
     ~StepTileTask() override;
 
     void execute(std::shared_ptr<HaloTile> tile) override;
+    // The update of execute() without emitting: replaces tile.data by the
+    // stepped tile.  For adapters onto other task runtimes.
+    void step(HaloTile& tile);
     std::shared_ptr<AbstractTask<HaloTile, HaloTile>> copy() override;
     void shutdown() override;
 
