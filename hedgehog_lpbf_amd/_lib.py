@@ -67,8 +67,16 @@ class StepRecord(C.Structure):
                 ("step", C.c_int), ("has_col", C.c_int)]
 
 
+class F32Instance(C.Structure):
+    """include/lpbf_batch.h lpbf_f32_instance (device pointers as integers)."""
+    _fields_ = [("k", StepConst), ("theta0", C.c_void_p), ("theta1", C.c_void_p),
+                ("ld", C.c_int), ("size_class", C.c_int), ("T_ref", C.c_double),
+                ("T_bc", C.c_double), ("Tm", C.c_double), ("d_plan", C.c_void_p),
+                ("d_stats", C.c_void_p), ("n_steps", C.c_int), ("pad_", C.c_int)]
+
+
 assert C.sizeof(StepConst) == 216 and C.sizeof(Partial) == 24
-assert C.sizeof(StepRecord) == 80
+assert C.sizeof(StepRecord) == 80 and C.sizeof(F32Instance) == 288
 assert C.sizeof(LaserTick) == 24 and C.sizeof(StepStats) == 56
 
 _vp, _i, _d, _v = C.c_void_p, C.c_int, C.c_double, None
@@ -106,6 +114,18 @@ SIGNATURES = {
     "lpbf_launch_theta_to_T": (_v, [_vp, _i, _i, _i, _d, _vp, _vp]),
     "lpbf_launch_T_to_theta": (_v, [_vp, _i, _i, _d, _vp, _i, _vp]),
     "lpbf_launch_extract_col_f32": (_v, [_vp, _i, _i, _i, _d, _vp, _vp, _vp]),
+    # batched FP32 instances (include/lpbf_batch.h)
+    "lpbf_f32_batch_create": (_i, [C.POINTER(F32Instance), _i, _i, C.POINTER(_vp)]),
+    "lpbf_f32_batch_destroy": (_v, [_vp]),
+    "lpbf_f32_batch_set_tiling": (_i, [_vp, _i, _i, _i]),
+    "lpbf_f32_batch_get_tiling": (_i, [_vp, _i, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]),
+    "lpbf_f32_batch_autotune": (_i, [_vp, _i, _vp, C.POINTER(C.c_double)]),
+    "lpbf_f32_batch_enqueue": (_i, [_vp, _i, _i, _vp]),
+    "lpbf_f32_batch_capture": (_i, [_vp, _i, _i, _vp]),
+    "lpbf_f32_batch_launch_graph": (_i, [_vp, _vp]),
+    "lpbf_f32_batch_current": (_i, [_vp, _i]),
+    "lpbf_f32_batch_launches": (_i, [_vp, _i, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     # device-resident run (include/lpbf_run.h)
     "lpbf_run_create": (_i, [_KP, _i, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _i,
                              C.POINTER(RunOptions), C.POINTER(_vp)]),

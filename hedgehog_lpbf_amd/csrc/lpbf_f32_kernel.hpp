@@ -1,0 +1,554 @@
+// SPDX-License-Identifier: 0BSD
+// FP32 performance path for gfx950: temporally blocked explicit-Euler steps
+// with fused per-step statistics.
+//
+// Field: theta = T - T_ref in FP32, rows of pitch `ld` floats (ld % 4 == 0,
+// 16-byte aligned base).  Storing the excess temperature keeps small
+// increments near ambient above FP32 resolution.
+//
+// Schedule (MI355X-first):
+//   * a wavefront owns a strip of 256 loaded columns (lane l holds 4
+//     contiguous columns as one 16-byte load) that overlaps its neighbours by
+//     kOv = 4 columns on each side, so 248 columns are written per strip;
+//   * the strip streams rows top to bottom through a K-level register
+//     pipeline: loading row i advances level j to row i-j, so one pass over
+//     HBM performs K time steps (K <= 4 keeps every written column valid,
+//     since the invalid overlap grows by one column per level);
+//   * x neighbours are cross-lane shuffles, y neighbours the register window;
+//     edge conditions are fixed up per row, only in edge strips;
+//   * per level, every owned cell-update feeds the step's statistics (peak T
+//     with its index, molten rows and count, energy sums): FP32 per lane,
+//     FP64 across lanes and workgroups;
+//   * four strips form a 256-thread workgroup; workgroup ids are remapped so
+//     consecutive tiles land on the same XCD (shared halo cache lines in L2).
+//
+// This header holds the tile body shared by the single-field kernel
+// (lpbf_f32.hip) and the multi-instance kernel (lpbf_f32_batch.hip), and the
+// host-side tiling geometry and argument set-up both use.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "lpbf_common.hpp"
+#include "lpbf_kernels.h"
+
+namespace lpbf_amd {
+namespace {
+
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / kWave;
+constexpr int kOv = 4;                          // overlap columns per side
+constexpr int kLoadCols = 4 * kWave;            // 256
+constexpr int kOutCols = kLoadCols - 2 * kOv;   // 248
+constexpr int kMaxDepth = 4;
+constexpr int kXcds = 8;
+constexpr int kPf = 2;                          // row prefetch distance
+
+struct F32Args {
+    const float* in;
+    float* out;
+    int nx, ny, ld;
+    int nstrips, R, ntasks, nwg;
+    int bc;
+    float cx, cy;          // alpha*dt/dx^2, alpha*dt/dy^2
+    // dt*loss(theta) in K = (((a4*th + a3)*th + a2)*th + a1)*th + a0 (Horner):
+    // convection + substrate are linear in theta, radiation expands
+    // (T_ref + theta)^4 - Tamb^4 around T_ref.
+    float a0, a1, a2, a3, a4;
+    float theta_m;         // Tm - T_ref
+    float theta_bc;        // T_bc - T_ref
+    float ax, ay;          // log2(e) * 2*dx^2/r0^2, log2(e) * 2*dy^2/r0^2
+    float cut;             // base-2 exponent beyond which the source is dropped
+    double inv_dx, inv_dy, cS0;  // 1/dx, 1/dy, dt*peak_S/(rho_cp*h)
+    double Tref_d;
+    StepStats* part;       // [depth][nwg]
+};
+
+__device__ inline int xcd_remap(int bid, int n) {
+    const int per = n / kXcds, rem = n % kXcds;
+    const int xcd = bid % kXcds, local = bid / kXcds;
+    return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+// Whole-wavefront lane shifts (DPP wave_shr:1 / wave_shl:1, GFX9 family):
+// lane l receives lane l-1 (from_left) or lane l+1 (from_right); the lane
+// without a source receives 0.
+__device__ __forceinline__ float from_left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float from_right(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+
+// Empty register constraint: marks the value as produced at this point
+// (emits no instruction).
+template <typename T>
+__device__ __forceinline__ void pin(T& v) {
+    asm volatile("" : "+v"(v));
+}
+
+__device__ __forceinline__ float& elem(float4& v, int e) {
+    return e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w));
+}
+
+// v[e] = t for a run-time e, as four selects (a reference into the vector
+// with a run-time index would put it in scratch memory).
+__device__ __forceinline__ void set_elem(float4& v, int e, float t) {
+    v.x = e == 0 ? t : v.x;
+    v.y = e == 1 ? t : v.y;
+    v.z = e == 2 ? t : v.z;
+    v.w = e == 3 ? t : v.w;
+}
+
+struct LevelAcc {
+    float bv;        // best theta of this lane
+    int bi;          // its row-major index (smallest on ties)
+    int ymin, ymax;  // molten rows
+    int molten;
+    double S;        // sum of theta over the owned cells after this step
+    float SL;        // sum of dt*loss
+    float SS;        // sum of dt*S
+};
+
+// Workgroup `wg` of one field: four strip tiles, K steps, and the
+// workgroup's per-level statistics records a.part[j*a.nwg + wg].
+template <int K, int BC, bool RAD>
+__device__ __forceinline__ void step_tile(const F32Args& a, const LaserTick* __restrict__ plan,
+                                          int step0, int wg) {
+    const int lane = threadIdx.x & (kWave - 1);
+    // wave-uniform: everything derived from the tile index lives in SGPRs
+    // and the row/strip conditions below are scalar branches
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int task = wg * kWaves + wave;
+    const int nx = a.nx, ny = a.ny;
+    constexpr float kNegInf = -3.0e38f;
+
+    // per level: laser position in cell units, source scale, and the rows
+    // [srlo, srhi] whose source can matter (|r - yc| below the cut-off, plus
+    // one row of margin); the row test is then a scalar compare
+    float xc[K], yc[K], cS[K];
+    int srlo[K], srhi[K];
+    const float wcut = sqrtf(a.cut / a.ay);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const LaserTick t = plan[step0 + j];
+        // uniform values: keep them in SGPRs
+        xc[j] = __int_as_float(__builtin_amdgcn_readfirstlane(
+            __float_as_int(static_cast<float>(t.x_l * a.inv_dx))));
+        yc[j] = __int_as_float(__builtin_amdgcn_readfirstlane(
+            __float_as_int(static_cast<float>(t.y_l * a.inv_dy))));
+        cS[j] = __int_as_float(__builtin_amdgcn_readfirstlane(
+            __float_as_int(static_cast<float>(a.cS0 * t.q_scale))));
+        srlo[j] = __builtin_amdgcn_readfirstlane(static_cast<int>(floorf(yc[j] - wcut)) - 1);
+        srhi[j] = __builtin_amdgcn_readfirstlane(static_cast<int>(ceilf(yc[j] + wcut)) + 1);
+        if (t.q_scale == 0.0) { srlo[j] = 1; srhi[j] = 0; }   // beam off: no source rows
+    }
+
+    LevelAcc acc[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc[j] = LevelAcc{kNegInf, 0x7fffffff, ny, -1, 0, 0.0, 0.f, 0.f};
+    double S0 = 0.0;   // sum of the input theta over the owned cells
+
+    if (task < a.ntasks) {
+        const int band = task / a.nstrips, strip = task % a.nstrips;
+        const int L0 = strip * kOutCols - kOv;
+        const int col4 = L0 + 4 * lane;
+        const bool ld_ok = col4 >= 0 && col4 < nx;
+        const bool own_lane = lane >= 1 && lane <= kWave - 2 && col4 < nx;
+        const int ya = band * a.R;
+        const int yb = min(ya + a.R, ny);
+        const int y_lo = max(ya - K, 0);
+        const int y_hi = min(yb + K, ny);
+        const int i_end = yb - 1 + K;   // level-K row yb-1 is produced at i = yb-1+K
+        const bool has_left = (L0 < 0);                  // column -1 at lane 0, elem 3
+        const int pR = nx - L0;                          // position of column nx
+        const bool has_right = (pR >= 0 && pR < kLoadCols);
+        const int padl = pR >> 2, pade = pR & 3;
+        const int pL = nx - 1 - L0;                      // position of column nx-1
+        const bool has_last = (pL >= 0 && pL < kLoadCols);
+        const int lastl = pL >> 2, laste = pL & 3;
+        // the lane that holds column nx-1 may also hold columns >= nx
+        const bool ragged = has_right && pade != 0;
+
+        // Mirror: column -1 := column 0, column nx := column nx-1.
+        auto fix_mirror = [&](float4& v) {
+            if (has_left) {
+                const float t = from_right(v.x);
+                if (lane == 0) v.w = t;
+            }
+            if (has_right) {
+                const float t = from_left(v.w);
+                if (lane == padl) {
+                    if (pade == 0) v.x = t;
+                    else if (pade == 1) v.y = v.x;
+                    else if (pade == 2) v.z = v.y;
+                    else v.w = v.z;
+                }
+            }
+        };
+        // Dirichlet: columns 0 and nx-1 := theta_bc; their loss leaves the sums.
+        auto fix_dirichlet = [&](float4& v, float4& L) {
+            if (has_left && lane == 1) { v.x = a.theta_bc; L.x = 0.f; }   // column 0
+            if (has_last && lane == lastl) {
+                set_elem(v, laste, a.theta_bc);
+                set_elem(L, laste, 0.f);
+            }
+        };
+        // Owned-cell view of a row for the reductions: non-owned lanes and
+        // columns >= nx drop out (-inf for the max, 0 for the sums).
+        auto row_sum = [&](float4 v) {
+            if (ragged && lane == lastl) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (e > laste) elem(v, e) = 0.f;
+            }
+            const float s = (v.x + v.y) + (v.z + v.w);
+            return own_lane ? s : 0.f;
+        };
+
+        auto own_sum = [&](float4 v) {
+            const float t = (v.x + v.y) + (v.z + v.w);
+            return own_lane ? t : 0.f;
+        };
+
+        auto load_row = [&](int i) {
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i < y_hi && ld_ok)
+                v = *reinterpret_cast<const float4*>(a.in + static_cast<long long>(i) * a.ld + col4);
+            return v;
+        };
+        // interior strips only (every lane in bounds): rows past y_hi are
+        // never consumed, so the row index is clamped instead of tested
+        auto load_row_interior = [&](int i) {
+            const int ii = min(i, y_hi - 1);
+            return *reinterpret_cast<const float4*>(a.in + static_cast<long long>(ii) * a.ld + col4);
+        };
+
+        // Register rings, indexed by a compile-time phase so the window
+        // advances without moves:
+        //   level 0: six slots holding rows i-2 .. i+3; at phase ph (mod 6)
+        //     row i (newest) is slot ph, i-1 is (ph+5)%6, i-2 is (ph+4)%6,
+        //     and row i+3 is loaded into slot (ph+3)%6, three rows ahead;
+        //   level j >= 1: three slots; at phase ph (mod 3) the newest row is
+        //     slot ph, the one before (ph+2)%3, the oldest (ph+1)%3.
+        // The steady-state loop runs six phases per trip (the common period),
+        // so every register keeps one role; the general body always runs at
+        // phase 0 and rotates afterwards.
+        float4 Z[6];
+        float4 W[K > 1 ? K - 1 : 1][3];   // W[j-1] is level j, j = 1..K-1
+#pragma unroll
+        for (int q = 0; q < 3; ++q) Z[q] = load_row(y_lo + q);
+
+        auto win_new = [&](auto PHC, auto JC) -> float4& {
+            constexpr int ph = decltype(PHC)::value, j = decltype(JC)::value;
+            if constexpr (j == 0) return Z[ph % 6];
+            else return W[j - 1][ph % 3];
+        };
+        auto win_mid = [&](auto PHC, auto JC) -> float4& {
+            constexpr int ph = decltype(PHC)::value, j = decltype(JC)::value;
+            if constexpr (j == 0) return Z[(ph + 5) % 6];
+            else return W[j - 1][(ph + 2) % 3];
+        };
+        auto win_old = [&](auto PHC, auto JC) -> float4& {
+            constexpr int ph = decltype(PHC)::value, j = decltype(JC)::value;
+            if constexpr (j == 0) return Z[(ph + 4) % 6];
+            else return W[j - 1][(ph + 1) % 3];
+        };
+
+        // One level-j row from level j-1's window.  GEN: any row of any strip
+        // (edge columns, edge rows, band-halo rows, mirror rows); otherwise
+        // an interior strip with every level on an owned interior row.
+        auto level = [&](auto PHC, auto JC, auto GENC, int i) {
+            constexpr int j = decltype(JC)::value;
+            constexpr bool GEN = decltype(GENC)::value;
+            using JP = std::integral_constant<int, j - 1>;
+            const int r = i - j;
+            if (GEN) {
+                const int lo = max(ya - (K - j), 0);
+                const int hi = min(yb + (K - j), ny);
+                if (r < lo || r >= hi) {
+                    if constexpr (j < K)
+                        if (r == ny) win_new(PHC, JC) = win_mid(PHC, JC);   // row ny := row ny-1
+                    return;
+                }
+            }
+            const float4 c = win_mid(PHC, JP{});
+            const float4 up = win_old(PHC, JP{});
+            const float4 dn = win_new(PHC, JP{});
+            const float wl = from_left(c.w);    // column left of c.x
+            const float wr = from_right(c.x);   // column right of c.w
+            const float cv[4] = {c.x, c.y, c.z, c.w};
+            const float xm[4] = {wl, c.x, c.y, c.z};
+            const float xp[4] = {c.y, c.z, c.w, wr};
+            const float uv[4] = {up.x, up.y, up.z, up.w};
+            const float dv[4] = {dn.x, dn.y, dn.z, dn.w};
+            float out4[4], loss4[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float th = cv[e];
+                float L;
+                if (RAD) L = fmaf(fmaf(fmaf(fmaf(a.a4, th, a.a3), th, a.a2), th, a.a1), th, a.a0);
+                else L = fmaf(a.a1, th, a.a0);
+                float d = fmaf(a.cx, fmaf(-2.f, th, xm[e] + xp[e]), -L);
+                d = fmaf(a.cy, fmaf(-2.f, th, uv[e] + dv[e]), d);
+                out4[e] = d;
+                loss4[e] = L;
+            }
+            // source: the whole row is skipped unless it is near the laser
+            float ssum = 0.f;
+            if (r >= srlo[j - 1] && r <= srhi[j - 1]) {
+                const float uy = static_cast<float>(r) - yc[j - 1];
+                const float ey = a.ay * uy * uy;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float ux = static_cast<float>(col4 + e) - xc[j - 1];
+                    // v_exp_f32: results below 2^-126 flush to 0, far under
+                    // FP32 resolution of any theta
+                    const float sv = cS[j - 1] * __builtin_amdgcn_exp2f(-fmaf(a.ax * ux, ux, ey));
+                    out4[e] += sv;
+                    ssum += (col4 + e < nx) ? sv : 0.f;
+                }
+                ssum = own_lane ? ssum : 0.f;
+            }
+            float4 res = make_float4(cv[0] + out4[0], cv[1] + out4[1], cv[2] + out4[2],
+                                     cv[3] + out4[3]);
+            float4 L4 = make_float4(loss4[0], loss4[1], loss4[2], loss4[3]);
+            if (GEN) {
+                if (BC == LPBF_BC_DIRICHLET) {
+                    if (r == 0 || r == ny - 1) {
+                        res = make_float4(a.theta_bc, a.theta_bc, a.theta_bc, a.theta_bc);
+                        L4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                        ssum = 0.f;
+                    } else {
+                        fix_dirichlet(res, L4);
+                    }
+                } else {
+                    fix_mirror(res);
+                }
+            }
+            if (!GEN || (r >= ya && r < yb)) {
+                if (j == K && own_lane)
+                    *reinterpret_cast<float4*>(a.out + static_cast<long long>(r) * a.ld + col4) = res;
+                LevelAcc& A = acc[j - 1];
+                float4 mx = res;
+                if (GEN && ragged && lane == lastl) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (e > laste) elem(mx, e) = kNegInf;
+                }
+                float m4 = fmaxf(fmaxf(mx.x, mx.y), fmaxf(mx.z, mx.w));
+                m4 = own_lane ? m4 : kNegInf;
+                if (m4 > A.bv) {
+                    A.bv = m4;
+                    const int e = (mx.x == m4) ? 0 : (mx.y == m4) ? 1 : (mx.z == m4) ? 2 : 3;
+                    A.bi = r * nx + col4 + e;
+                }
+                if (m4 > a.theta_m) {
+                    A.molten += (mx.x > a.theta_m) + (mx.y > a.theta_m) + (mx.z > a.theta_m) +
+                                (mx.w > a.theta_m);
+                    A.ymin = min(A.ymin, r);
+                    A.ymax = r;
+                }
+                A.S += static_cast<double>(GEN ? row_sum(res) : own_sum(res));
+                A.SL += GEN ? row_sum(L4) : own_sum(L4);
+                A.SS += ssum;
+                // keep each running sum updated here: left alone the compiler
+                // defers the serial adds to the end of the unrolled trip and
+                // holds every phase's partial live (about 60 extra VGPRs)
+                pin(A.S);
+                pin(A.SL);
+                pin(A.SS);
+            }
+            if constexpr (j < K) {
+                win_new(PHC, JC) = res;
+                if (GEN && r == 0) win_mid(PHC, JC) = res;   // row -1 := row 0
+            }
+        };
+
+        using J0 = std::integral_constant<int, 0>;
+        auto body = [&](auto PHC, auto GENC, int i) {
+            constexpr int ph = decltype(PHC)::value;
+            constexpr bool GEN = decltype(GENC)::value;
+            Z[(ph + 3) % 6] = GEN ? load_row(i + 3) : load_row_interior(i + 3);
+            float4& v = win_new(PHC, J0{});
+            if (!GEN || i < y_hi) {
+                if (GEN && BC == LPBF_BC_MIRROR) fix_mirror(v);
+                if (i >= ya && i < yb) {
+                    S0 += static_cast<double>(GEN ? row_sum(v) : own_sum(v));
+                    pin(S0);
+                }
+            } else if (i == ny) {
+                v = win_mid(PHC, J0{});                 // row ny := row ny-1
+            }
+            if (GEN && i == 0) win_mid(PHC, J0{}) = v;  // row -1 := row 0
+            // levels run one after the other: the next level needs this
+            // level's row anyway, and interleaving them only raises pressure
+            level(PHC, std::integral_constant<int, 1>{}, GENC, i);
+            __builtin_amdgcn_sched_barrier(0);
+            if constexpr (K >= 2) level(PHC, std::integral_constant<int, 2>{}, GENC, i);
+            __builtin_amdgcn_sched_barrier(0);
+            if constexpr (K >= 3) level(PHC, std::integral_constant<int, 3>{}, GENC, i);
+            __builtin_amdgcn_sched_barrier(0);
+            if constexpr (K >= 4) level(PHC, std::integral_constant<int, 4>{}, GENC, i);
+        };
+        using Gen = std::true_type;
+        using Fast = std::false_type;
+        auto general = [&](int i) {
+            body(std::integral_constant<int, 0>{}, Gen{}, i);
+            // back to the phase-0 layout
+            const float4 z0 = Z[0];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) Z[q] = Z[q + 1];
+            Z[5] = z0;
+#pragma unroll
+            for (int q = 0; q + 1 < K; ++q) { W[q][1] = W[q][2]; W[q][2] = W[q][0]; }
+        };
+
+        // steady state: interior strip, every level on an owned row that is
+        // neither an edge row nor next to one
+        const bool interior = !has_left && !has_right && !has_last;
+        const int i_s = interior ? max(ya + K, K + 1) : i_end + 1;
+        const int i_e = interior ? min(yb + 1, ny - 1) : i_end + 1;
+        int i = y_lo;
+        for (; i <= i_end && i < i_s; ++i) general(i);
+        // scheduling barriers keep the phases apart: interleaving them buys
+        // no latency the two-plus resident waves do not already hide, and
+        // costs registers
+        for (; i + 6 <= i_e; i += 6) {
+            body(std::integral_constant<int, 0>{}, Fast{}, i);
+            __builtin_amdgcn_sched_barrier(0);
+            body(std::integral_constant<int, 1>{}, Fast{}, i + 1);
+            __builtin_amdgcn_sched_barrier(0);
+            body(std::integral_constant<int, 2>{}, Fast{}, i + 2);
+            __builtin_amdgcn_sched_barrier(0);
+            body(std::integral_constant<int, 3>{}, Fast{}, i + 3);
+            __builtin_amdgcn_sched_barrier(0);
+            body(std::integral_constant<int, 4>{}, Fast{}, i + 4);
+            __builtin_amdgcn_sched_barrier(0);
+            body(std::integral_constant<int, 5>{}, Fast{}, i + 5);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        for (; i <= i_end; ++i) general(i);
+    }
+
+    // per level: wavefront then workgroup merge -> a.part[j*nwg + wg]
+    __shared__ PartialAcc s_pa[kWaves];
+    __shared__ double s_sum[4][kWaves];
+    __shared__ long long s_m[kWaves];
+    const long long N = static_cast<long long>(nx) * ny;
+    double prevS = wave_sum(S0);   // lane 0 holds the wavefront sums
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const LevelAcc& A = acc[j];
+        PartialAcc p;
+        p.tmax = (A.bi == 0x7fffffff) ? -1.0 : a.Tref_d + static_cast<double>(A.bv);
+        p.arg = (A.bi == 0x7fffffff) ? N : static_cast<long long>(A.bi);
+        p.ymin = A.ymin;
+        p.ymax = A.ymax;
+        const PartialAcc w = wave_reduce(p);
+        const double wS = wave_sum(A.S);
+        const double ws = wave_sum(static_cast<double>(A.SS));
+        const double wl = wave_sum(static_cast<double>(A.SL));
+        const long long wm = wave_sum(static_cast<long long>(A.molten));
+        if (lane == 0) {
+            s_pa[wave] = w;
+            s_sum[0][wave] = wS - prevS;   // sum of (theta' - theta) over the owned cells
+            s_sum[1][wave] = ws;
+            s_sum[2][wave] = wl;
+            s_m[wave] = wm;
+        }
+        prevS = wS;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            PartialAcc b = s_pa[0];
+            StepStats st;
+            st.molten = s_m[0]; st.sum_dT = s_sum[0][0]; st.sum_src = s_sum[1][0];
+            st.sum_loss = s_sum[2][0];
+            for (int q = 1; q < kWaves; ++q) {
+                b.merge(s_pa[q]);
+                st.molten += s_m[q]; st.sum_dT += s_sum[0][q];
+                st.sum_src += s_sum[1][q]; st.sum_loss += s_sum[2][q];
+            }
+            st.tmax = b.tmax; st.arg = b.arg; st.ymin = b.ymin; st.ymax = b.ymax;
+            a.part[static_cast<long long>(j) * a.nwg + wg] = st;
+        }
+        __syncthreads();
+    }
+}
+
+// Built-in band height: about six resident wavefront tiles per SIMD (1024
+// SIMDs) so latency is hidden, but never below 8*depth rows, where the
+// 2*depth recomputed halo rows would dominate.  lpbf_f32_autotune measures
+// instead of guessing.
+inline int rows_per_band(int nx, int ny, int depth) {
+    const long long nstrips = (nx + kOutCols - 1) / kOutCols;
+    const long long bands = (6 * 1024 + nstrips - 1) / nstrips;
+    int R = static_cast<int>((ny + bands - 1) / bands);
+    const int lo = depth > 1 ? 8 * depth : 16;
+    if (R < lo) R = lo;
+    if (R > ny) R = ny;
+    return R < 1 ? 1 : R;
+}
+
+struct Geometry {
+    int nstrips, R, ntasks, nwg;
+};
+
+inline Geometry geometry(int nx, int ny, int depth, int rows) {
+    Geometry g;
+    g.nstrips = (nx + kOutCols - 1) / kOutCols;
+    g.R = rows > 0 ? rows : rows_per_band(nx, ny, depth);
+    g.ntasks = g.nstrips * ((ny + g.R - 1) / g.R);
+    g.nwg = (g.ntasks + kWaves - 1) / kWaves;
+    return g;
+}
+
+// The FP32 kernels index cells with 32-bit ints and rely on nx, ny for every
+// address; shapes outside this range are rejected.
+inline bool f32_shape_ok(long long nx, long long ny) {
+    return nx >= 2 && ny >= 1 && nx <= (1LL << 28) && ny <= (1LL << 28) &&
+           nx * ny < (1LL << 31) - 1;
+}
+
+inline F32Args make_args(const float* in, float* out, int ld, const StepConst& k, double T_ref,
+                         int bc, double T_bc, double Tm, int depth, int rows, void* work) {
+    F32Args a{};
+    const Geometry g = geometry(k.nx, k.ny, depth, rows);
+    a.in = in; a.out = out; a.nx = k.nx; a.ny = k.ny; a.ld = ld;
+    a.nstrips = g.nstrips; a.R = g.R; a.ntasks = g.ntasks; a.nwg = g.nwg;
+    a.bc = bc;
+    const double dt = k.dt;
+    a.cx = static_cast<float>(k.alpha * dt * k.inv_dx2);
+    a.cy = static_cast<float>(k.alpha * dt * k.inv_dy2);
+    // dt*loss in K as a polynomial in theta = T - T_ref:
+    //   cH*(theta + T_ref - Tamb) + cB*(theta + T_ref - Tsub)
+    //   + cR*((T_ref + theta)^4 - Tamb^4)
+    const double cH = dt * k.hConv * k.inv_h / k.rho_cp;
+    const double cB = dt * (k.K / k.dSub) * k.inv_h / k.rho_cp;
+    const double cR = k.radiation ? dt * k.eps * 5.670374419e-8 * k.inv_h / k.rho_cp : 0.0;
+    const double Tr = T_ref, Ta = k.Tamb;
+    a.a0 = static_cast<float>(cH * (Tr - Ta) + cB * (Tr - k.Tsub) +
+                              cR * (Tr * Tr * Tr * Tr - Ta * Ta * Ta * Ta));
+    a.a1 = static_cast<float>(cH + cB + 4.0 * cR * Tr * Tr * Tr);
+    a.a2 = static_cast<float>(6.0 * cR * Tr * Tr);
+    a.a3 = static_cast<float>(4.0 * cR * Tr);
+    a.a4 = static_cast<float>(cR);
+    a.theta_m = static_cast<float>(Tm - T_ref);
+    a.theta_bc = static_cast<float>(T_bc - T_ref);
+    const double log2e = 1.4426950408889634;
+    a.ax = static_cast<float>(log2e * 2.0 * k.dx * k.dx * k.inv_r0_sq);
+    a.ay = static_cast<float>(log2e * 2.0 * k.dy * k.dy * k.inv_r0_sq);
+    a.cut = static_cast<float>(log2e * 64.0);   // exp(-64) ~ 1.6e-28 of the peak
+    a.inv_dx = 1.0 / k.dx;
+    a.inv_dy = 1.0 / k.dy;
+    a.cS0 = dt * k.peak_S / (k.rho_cp * k.h);
+    a.Tref_d = T_ref;
+    a.part = static_cast<StepStats*>(work);
+    return a;
+}
+
+}  // namespace
+}  // namespace lpbf_amd

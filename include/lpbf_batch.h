@@ -1,0 +1,102 @@
+/* SPDX-License-Identifier: 0BSD
+ *
+ * Batched FP32 instances (no reference counterpart): many process-parameter
+ * instances of uneven grid size advanced together, each launch stepping every
+ * instance of a group by `depth` steps with one multi-instance kernel, the
+ * whole run capturable as one hipGraph.
+ *
+ * The reference runs such sweeps one after another (src/gpu/hh_gpu_main.cpp
+ * run_manifest, :234-290, one graph and one device field per row); here
+ * every instance is resident at once and the launch grid covers all of them.
+ *
+ * Instances carry a size class; instances of one class share the temporal
+ * depth (1..4) and band height, chosen by lpbf_f32_batch_set_tiling or
+ * measured by lpbf_f32_batch_autotune.  Classes with equal depth (and equal
+ * radiation switch) share a launch.
+ *
+ * Fields follow the lpbf_launch_step_f32 layout (theta = T - T_ref, pitch ld,
+ * ld % 4 == 0, 16-byte aligned).  Step s of instance i uses d_plan[s] of that
+ * instance and, if d_stats != NULL, writes d_stats[s].  Each instance
+ * alternates between theta0 and theta1; lpbf_f32_batch_current tells which
+ * holds its latest field.
+ *
+ * Return values: 0 or an error code (hipError_t value or
+ * LPBF_ERR_INVALID_ARGUMENT), also recorded for lpbf_get_last_error().
+ * Calls on one batch must not overlap.
+ */
+#ifndef LPBF_BATCH_H
+#define LPBF_BATCH_H
+
+#include <hip/hip_runtime_api.h>
+#include "lpbf_step_const.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lpbf_f32_instance {
+    StepConst        k;           /* grid and physics; x_l, y_l, q_scale unused */
+    float*           theta0;      /* initial field (device) */
+    float*           theta1;      /* second buffer (device) */
+    int              ld;          /* row pitch in floats */
+    int              size_class;  /* >= 0 */
+    double           T_ref;       /* theta origin [K] */
+    double           T_bc;        /* Dirichlet edge temperature [K] */
+    double           Tm;          /* melting point for the statistics [K] */
+    const LaserTick* d_plan;      /* laser per step (device), n_steps ticks */
+    StepStats*       d_stats;     /* per-step statistics (device), n_steps records, or NULL */
+    int              n_steps;     /* length of d_plan (and d_stats) */
+    int              pad_;
+} lpbf_f32_instance;
+
+typedef struct lpbf_f32_batch lpbf_f32_batch;
+
+/* bc: LPBF_BC_MIRROR or LPBF_BC_DIRICHLET for every instance.  The batch
+ * keeps a copy of the descriptors; the device buffers stay the caller's. */
+int  lpbf_f32_batch_create(const lpbf_f32_instance* inst, int n_inst, int bc,
+                           lpbf_f32_batch** out);
+void lpbf_f32_batch_destroy(lpbf_f32_batch* b);
+
+/* Tiling of one size class: depth 1..4, rows = band height (0 = built-in).
+ * Changing the tiling (or autotuning) discards a captured graph. */
+int  lpbf_f32_batch_set_tiling(lpbf_f32_batch* b, int size_class, int depth, int rows);
+int  lpbf_f32_batch_get_tiling(const lpbf_f32_batch* b, int size_class, int* depth,
+                               int* rows, int* n_members);
+
+/* Times every class alone for depths 2..max_depth (1 if max_depth == 1)
+ * and a set of band heights with the instances' own plans from step 0 and
+ * keeps the fastest per class.  The fields are saved and restored
+ * (synchronises `stream`); *ms_per_step (may be NULL) receives the sum over
+ * classes of the best time per step. */
+int  lpbf_f32_batch_autotune(lpbf_f32_batch* b, int max_depth, hipStream_t stream,
+                             double* ms_per_step);
+
+/* Enqueues steps [step0, step0+n_steps) of every instance on `stream`
+ * (asynchronous, no host synchronisation; may be captured).  The steps must
+ * lie within every instance's plan. */
+int  lpbf_f32_batch_enqueue(lpbf_f32_batch* b, int step0, int n_steps, hipStream_t stream);
+
+/* Captures the launches of lpbf_f32_batch_enqueue(step0, n_steps) as one
+ * hipGraph (nothing runs).  Every group must have an even number of launches
+ * so that replays leave each field in the buffer it started in; a schedule
+ * with an odd count is made even by splitting its last multi-step launch,
+ * and an odd count of single-step launches is refused. */
+int  lpbf_f32_batch_capture(lpbf_f32_batch* b, int step0, int n_steps, hipStream_t stream);
+/* Launches the captured graph on `stream` (may be repeated). */
+int  lpbf_f32_batch_launch_graph(lpbf_f32_batch* b, hipStream_t stream);
+
+/* 0 if theta0, 1 if theta1 holds instance i's latest field; < 0 on error. */
+int  lpbf_f32_batch_current(const lpbf_f32_batch* b, int instance);
+
+/* Workgroups and launches per enqueue of n_steps (for traffic accounting). */
+int  lpbf_f32_batch_launches(const lpbf_f32_batch* b, int n_steps, int* n_launches,
+                             long long* cells_per_step);
+
+#ifdef __cplusplus
+}
+static_assert(sizeof(lpbf_f32_instance) == 288, "lpbf_f32_instance layout");
+#else
+_Static_assert(sizeof(lpbf_f32_instance) == 288, "lpbf_f32_instance layout");
+#endif
+
+#endif /* LPBF_BATCH_H */

@@ -67,6 +67,24 @@ def test_run_create_rejects_bad_options_before_touching_a_device():
     assert h.lib().lpbf_get_last_error() == 0
 
 
+def test_batch_create_rejects_bad_instances_before_touching_a_device():
+    lib = h.lib()
+    arr = (h._lib.F32Instance * 2)()
+    for a in arr:
+        a.k.nx, a.k.ny, a.k.dt = 64, 32, 1e-6
+        a.theta0, a.theta1, a.ld, a.d_plan, a.n_steps = 256, 512, 64, 1024, 10
+    out = C.c_void_p()
+    assert lib.lpbf_f32_batch_create(arr, 2, 7, C.byref(out)) == 1000          # bad bc
+    arr[1].ld = 62                                                           # ld < nx
+    assert lib.lpbf_f32_batch_create(arr, 2, 0, C.byref(out)) == 1000
+    arr[1].ld, arr[1].theta1 = 64, 256                                       # aliasing
+    assert lib.lpbf_f32_batch_create(arr, 2, 0, C.byref(out)) == 1000
+    assert lib.lpbf_f32_batch_create(arr, 0, 0, C.byref(out)) == 1000
+    assert lib.lpbf_get_last_error() == 1000
+    assert lib.lpbf_f32_batch_enqueue(None, 0, 1, None) == 1000
+    lib.lpbf_get_last_error()
+
+
 def test_make_const_and_dt_equal_oracle(ora):
     for case in ("0", "1.1", "3.2"):
         s = ora.defaults(case)
