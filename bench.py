@@ -7,7 +7,10 @@ Workload (default, `--workload plate8k_f32`): an 8192 x 8192 plate, dx = dy
 steps are its first K), mirror edges, FP32 field (theta = T - 300 K) with the
 fused per-step statistics (peak T + index, molten rows and count, energy
 sums) in FP64 across the grid.  One bench step = one time step of the whole
-plate.  `--workload plate8k_f64` runs the same plate on the FP64 parity path.
+plate.  `--workload plate8k_f64` runs the same plate on the FP64 parity path;
+`batch256` steps 256 process-parameter instances of uneven size together
+(one bench step = one time step of every instance); `case0` times the
+reference's published case on the device-resident FP64 path.
 
 Prints one JSON line: value = cell-updates/s over the timed steps, with the
 inputs already resident in HBM; `roofline` for the dominant kernel (step
@@ -76,6 +79,166 @@ def traffic_for(workload):
     return None
 
 
+def batch_setup(n_inst=256, n_steps=5000, seed=0):
+    """256 instances, sides log-uniform in [256, 2048] per axis, P ~ U[100, 400] W,
+    V ~ U[0.5, 2] m/s, r0 ~ U[35, 70] um, dx = 10 um, dt = 0.8 * dx^2 / (4 alpha),
+    single track at mid-height with the reference wrap law, T0 = 300 K."""
+    import numpy as np
+
+    import hedgehog_lpbf_amd as h
+    rng = np.random.default_rng(seed)
+    sides = np.rint(np.exp(rng.uniform(np.log(256), np.log(2048), size=(n_inst, 2)))).astype(int)
+    P = rng.uniform(100.0, 400.0, n_inst)
+    V = rng.uniform(0.5, 2.0, n_inst)
+    r0 = rng.uniform(35e-6, 70e-6, n_inst)
+    mat = h.Material(k=7.0, rho=4430.0, cp=526.0, Tm=1609.0, eps=0.40, eta=0.35)
+    dx = 1e-5
+    dt = 0.8 * dx * dx / (4.0 * mat.alpha())
+    specs = []
+    for i in range(n_inst):
+        nx, ny = int(sides[i, 0]), int(sides[i, 1])
+        pr = h.Process(P=float(P[i]), V=float(V[i]), r0=float(r0[i]), h=50e-6, Tamb=300.0,
+                       Tsub=300.0)
+        dom = h.Domain(Lx=(nx - 1) * dx, Ly=(ny - 1) * dx, nx=nx, ny=ny)
+        k = h.make_const(mat, pr, dom, dt)
+        plan = h.single_track_plan(pr.V, dt, dom.Lx, dom.Ly, n_steps)[0]
+        cells = nx * ny
+        cls = 0 if cells < (1 << 18) else (1 if cells < (1 << 20) else 2)
+        specs.append(h.InstanceSpec(k=k, plan=plan, T0=300.0, T_ref=300.0, T_bc=300.0,
+                                    Tm=mat.Tm, size_class=cls))
+    return specs, dt
+
+
+def batch_bench(args):
+    """All instances advance together: the autotuner picks depth and band height
+    per size class, the warm-up runs eagerly, the timed steps are one hipGraph
+    launch of the whole schedule."""
+    import numpy as np
+    import torch
+
+    import hedgehog_lpbf_amd as h
+    from oracle import pyoracle as ora
+    steps = args.steps if args.steps is not None else 5000
+    warmup = args.warmup if args.warmup is not None else 200
+    specs, dt = batch_setup(n_steps=max(steps, warmup, 4))
+    cells_total = sum(sp.k.nx * sp.k.ny for sp in specs)
+    print(f"[bench] batch256: {len(specs)} instances, {cells_total} cells, dt {dt:.4g} s, "
+          f"warmup={warmup} steps={steps}", file=sys.stderr, flush=True)
+    b = h.F32Batch(specs, bc=0)
+    s = torch.cuda.Stream()
+    t0 = time.perf_counter()
+    ms_tuned = b.autotune(4, s.cuda_stream)
+    tiling = {c: b.tiling(c) for c in (0, 1, 2)}
+    print(f"[bench] autotune {time.perf_counter() - t0:.1f} s: {tiling} "
+          f"({ms_tuned:.3f} ms/step summed over classes)", file=sys.stderr, flush=True)
+    with torch.cuda.stream(s):
+        b.enqueue(0, warmup, s.cuda_stream)
+        b.capture(0, steps, s.cuda_stream)
+    torch.cuda.synchronize()
+    h.check()
+    n_launch, cells = b.launches(steps)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(s)
+    with torch.cuda.stream(s):
+        b.launch_graph(s.cuda_stream)
+    e1.record(s)
+    torch.cuda.synchronize()
+    h.check()
+    total_ms = e0.elapsed_time(e1)
+    value = cells * steps / (total_ms * 1e-3)
+    peak = max(st.tmax for st in b.stats(0, 0, steps))
+    print(f"[bench] graph of {n_launch} step launches: {total_ms:.1f} ms, instance 0 peak T "
+          f"{peak:.1f} K", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        lib = ora.lib()
+        done = 0
+        n_inst = 0
+        t0 = time.perf_counter()
+        for sp in specs:
+            kk = ora.StepConst.from_buffer_copy(bytes(sp.k))
+            T = np.full((sp.k.ny, sp.k.nx), 300.0)
+            out = np.empty_like(T)
+            for q in range(10):
+                kk.x_l, kk.y_l, kk.q_scale = sp.plan[q]
+                lib.ora_step(ora._dp(T), ora._dp(out), C.byref(kk), 0)
+                T, out = out, T
+            done += 10 * sp.k.nx * sp.k.ny
+            n_inst += 1
+            if time.perf_counter() - t0 > 12.0:
+                break
+        el = time.perf_counter() - t0
+        cpu = {"value": done / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"oracle FP64 step (C, -O2, 1 thread), first {n_inst} instances x 10 "
+                         f"steps, {el:.1f} s"}
+    achieved = 8.0 * cells * steps / (total_ms * 1e-3) / 1e9
+    out = {
+        "metric": "cell-updates/s", "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": total_ms / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(specs)} instances, sides log-uniform 256-2048, P U[100,400] W, "
+                        f"V U[0.5,2] m/s, r0 U[35,70] um, dx 10 um, dt 0.8x stability, "
+                        f"mirror edges, FP32, per-step stats, one hipGraph launch",
+            "cells_per_step": cells, "instances": len(specs), "graph_launches": n_launch,
+            "tiling_per_class": {str(c): {"depth": d, "rows": r, "members": m}
+                                 for c, (d, r, m) in tiling.items()},
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": "k_step_f32_multi (whole graph incl. stats merges)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for("batch256"),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    b.close()
+
+
+def case0_bench(args):
+    """The reference's own published run (case0, 240x120, 1427 steps, FP64) on the
+    device-resident path: field on the GPU, one host sync per 64-step batch,
+    per-step metrics as the reference drivers compute them."""
+    import torch
+
+    import hedgehog_lpbf_amd as h
+    mat, pr, dom = h.Material(), h.apply_case("0", h.Process()), h.Domain()
+    dt = h.dt_cfl(mat, dom)
+    n = int(1.5e-3 / dt)
+    k = h.make_const(mat, pr, dom, dt)
+    plan, x_csv, wrapped = h.single_track_plan(pr.V, dt, dom.Lx, dom.Ly, n)
+    reps = args.steps if args.steps is not None else 20
+    for _ in range(args.warmup if args.warmup is not None else 2):
+        with h.DeviceRun(k, Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, Tamb=pr.Tamb, plan=plan,
+                         x_csv=x_csv, wrapped=wrapped, batch=64) as r:
+            r.advance(n)
+    torch.cuda.synchronize()
+    walls = []
+    for _ in range(reps):
+        with h.DeviceRun(k, Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, Tamb=pr.Tamb, plan=plan,
+                         x_csv=x_csv, wrapped=wrapped, batch=64) as r:
+            t0 = time.perf_counter()
+            r.advance(n)
+            walls.append(time.perf_counter() - t0)
+            peak_T = r.summary()[0]
+    wall = sorted(walls)[len(walls) // 2]
+    value = dom.nx * dom.ny * n / wall
+    ref_value = dom.nx * dom.ny * n / 0.198      # reference/case0/meta.json wall_ms 198
+    print(json.dumps({
+        "metric": "cell-updates/s", "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": n, "warmup": 0, "ms_per_step": wall * 1e3 / n, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": value / ref_value, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "reference case0: 240x120, 1427 steps, FP64 parity kernels, "
+                               "device-resident run, batch 64, per-step metrics",
+                   "median_of": reps, "peak_T_K": peak_T,
+                   "reference_cpu_wall_ms": 198.0},
+        "roofline": None, "cpu_baseline": None}), flush=True)
+
+
 def copy_reference(args):
     """Plain device-to-device copy of an n x n FP32 plate, `steps` times."""
     import torch
@@ -101,12 +264,16 @@ def copy_reference(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 4000 for the plates, 5000 for batch256, "
+                         "20 repetitions for case0)")
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="plate8k_f32",
-                    choices=["plate8k_f32", "plate8k_f64", "copy"],
-                    help="copy: a plain device copy of one n x n FP32 plate per step, the "
-                         "reference for calibrating HBM byte counters")
+                    choices=["plate8k_f32", "plate8k_f64", "batch256", "case0", "copy"],
+                    help="batch256: 256 uneven instances in one hipGraph launch; case0: the "
+                         "reference's published case on the device-resident path; copy: a "
+                         "plain device copy of one n x n FP32 plate per step, the reference "
+                         "for calibrating HBM byte counters")
     ap.add_argument("--depth", type=int, default=0,
                     help="FP32 steps per launch (1-4); 0 = pick depth and band rows with "
                          "the library's autotuner before the warm-up")
@@ -125,6 +292,14 @@ def main():
     import hedgehog_lpbf_amd as h
 
     lib = h.lib()
+    if args.workload == "batch256":
+        return batch_bench(args)
+    if args.workload == "case0":
+        return case0_bench(args)
+    if args.steps is None:
+        args.steps = 4000 if args.workload != "copy" else 10
+    if args.warmup is None:
+        args.warmup = 400 if args.workload != "copy" else 2
     if args.workload == "copy":
         return copy_reference(args)
     mat, pr, dom, dt, k, plan = plate_setup(args.n, args.n)

@@ -11,7 +11,7 @@ template <int K, int BC, bool RAD>
 __global__ __launch_bounds__(kThreads) void k_step_f32(F32Args a,
                                                        const LaserTick* __restrict__ plan,
                                                        int step0) {
-    step_tile<K, BC, RAD>(a, plan, step0, xcd_remap(blockIdx.x, a.nwg));
+    step_tile<K, BC, RAD>(a, nullptr, nullptr, plan, step0, xcd_remap(blockIdx.x, a.nwg));
 }
 
 // One workgroup per level merges that level's workgroup records.

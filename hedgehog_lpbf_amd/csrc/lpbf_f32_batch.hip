@@ -24,7 +24,7 @@ namespace lpbf_amd {
 namespace {
 
 struct InstArgs {
-    F32Args a;                 // a.in / a.out are taken from f[] per launch
+    F32Args a;                 // a.in / a.out unused: the buffers are f[parity], f[parity^1]
     float* f[2];
     const LaserTick* plan;
     StepStats* stats;          // per-step records or null
@@ -39,10 +39,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f32_multi(const InstArgs* __r
                                                              int step0) {
     const int g = xcd_remap(blockIdx.x, total_wg);
     const InstArgs& I = inst[wg_inst[g]];
-    F32Args a = I.a;
-    a.in = I.f[parity];
-    a.out = I.f[parity ^ 1];
-    step_tile<K, BC, RAD>(a, I.plan, step0, g - I.wg0);
+    step_tile<K, BC, RAD, true>(I.a, I.f[parity], I.f[parity ^ 1], I.plan, step0, g - I.wg0);
 }
 
 // Block (i, j): merge level j of instance i into its record for step step0+j.

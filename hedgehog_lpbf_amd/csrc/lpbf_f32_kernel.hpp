@@ -37,7 +37,6 @@
 namespace lpbf_amd {
 namespace {
 
-
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / kWave;
 constexpr int kOv = 4;                          // overlap columns per side
@@ -114,10 +113,16 @@ struct LevelAcc {
 };
 
 // Workgroup `wg` of one field: four strip tiles, K steps, and the
-// workgroup's per-level statistics records a.part[j*a.nwg + wg].
-template <int K, int BC, bool RAD>
-__device__ __forceinline__ void step_tile(const F32Args& a, const LaserTick* __restrict__ plan,
-                                          int step0, int wg) {
+// workgroup's per-level statistics records a.part[j*a.nwg + wg].  The field
+// moves from a.in to a.out, or, with BUFS, from src to dst: the
+// multi-instance kernel passes its buffers directly and keeps `a` in memory,
+// where its fields are loaded at their uses (a local copy of `a` costs nine
+// more VGPRs and one wave per SIMD at K = 4; the kernel-argument form is
+// allocated best reading a.in / a.out itself).
+template <int K, int BC, bool RAD, bool BUFS = false>
+__device__ __forceinline__ void step_tile(const F32Args& a, const float* __restrict__ src,
+                                          float* __restrict__ dst,
+                                          const LaserTick* __restrict__ plan, int step0, int wg) {
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform: everything derived from the tile index lives in SGPRs
     // and the row/strip conditions below are scalar branches
@@ -217,14 +222,14 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const LaserTick* __r
         auto load_row = [&](int i) {
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
             if (i < y_hi && ld_ok)
-                v = *reinterpret_cast<const float4*>(a.in + static_cast<long long>(i) * a.ld + col4);
+                v = *reinterpret_cast<const float4*>((BUFS ? src : a.in) + static_cast<long long>(i) * a.ld + col4);
             return v;
         };
         // interior strips only (every lane in bounds): rows past y_hi are
         // never consumed, so the row index is clamped instead of tested
         auto load_row_interior = [&](int i) {
             const int ii = min(i, y_hi - 1);
-            return *reinterpret_cast<const float4*>(a.in + static_cast<long long>(ii) * a.ld + col4);
+            return *reinterpret_cast<const float4*>((BUFS ? src : a.in) + static_cast<long long>(ii) * a.ld + col4);
         };
 
         // Register rings, indexed by a compile-time phase so the window
@@ -331,7 +336,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const LaserTick* __r
             }
             if (!GEN || (r >= ya && r < yb)) {
                 if (j == K && own_lane)
-                    *reinterpret_cast<float4*>(a.out + static_cast<long long>(r) * a.ld + col4) = res;
+                    *reinterpret_cast<float4*>((BUFS ? dst : a.out) + static_cast<long long>(r) * a.ld + col4) = res;
                 LevelAcc& A = acc[j - 1];
                 float4 mx = res;
                 if (GEN && ragged && lane == lastl) {
