@@ -126,6 +126,17 @@ SIGNATURES = {
     "lpbf_f32_batch_launch_graph": (_i, [_vp, _vp]),
     "lpbf_f32_batch_current": (_i, [_vp, _i]),
     "lpbf_f32_batch_launches": (_i, [_vp, _i, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
+    # surrogate (include/lpbf_surrogate.h)
+    "lpbf_sg_pack_bytes": (_sz, []),
+    "lpbf_sg_work_bytes": (_sz, [_i, _i]),
+    "lpbf_sg_pack": (_v, [_vp, _vp, _vp]),
+    "lpbf_sg_forward": (_v, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "lpbf_sg_loss_grad": (_v, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "lpbf_sg_adam": (_v, [_vp, _vp, _vp, _vp, _i, _i, C.c_float, C.c_float, C.c_float,
+                          C.c_float, _vp]),
+    "lpbf_sg_gather": (_v, [_vp, _i, _vp, _vp, _vp]),
+    "lpbf_sg_target": (_v, [_vp, _i, _vp, _vp, _vp]),
+    "lpbf_sg_apply": (_v, [_vp, _i, _vp, _vp, _i, _vp, _vp, _vp]),
     # device-resident run (include/lpbf_run.h)
     "lpbf_run_create": (_i, [_KP, _i, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _i,
                              C.POINTER(RunOptions), C.POINTER(_vp)]),

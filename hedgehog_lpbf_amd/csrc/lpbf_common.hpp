@@ -66,6 +66,13 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+// Lane 0 receives the wavefront maximum.
+__device__ inline float wave_max_f(float v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off));
+    return v;
+}
+
 __device__ inline long long wave_sum(long long v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off);

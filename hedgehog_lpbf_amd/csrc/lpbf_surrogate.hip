@@ -1,0 +1,761 @@
+// SPDX-License-Identifier: 0BSD
+// Melt-pool surrogate kernels for gfx950 (include/lpbf_surrogate.h).
+//
+// Activations are NHWC bf16 with 32 channels per pixel (64 bytes), so one
+// 16-byte load gives a lane the 8 consecutive channels an MFMA operand needs.
+//
+//   k_conv_mfma  conv3x3 32 -> 32 as an implicit GEMM on
+//                v_mfma_f32_16x16x32_bf16: per tap, D[co][px] += W[co][ci] *
+//                X[px][ci]; a wavefront owns 64 pixels of one row and all 32
+//                output channels (8 accumulator tiles), weights stay in
+//                registers for all 9 taps.  Epilogue: bias + gelu (forward,
+//                z and a stored) or the gelu' factor of the previous layer
+//                (backward data, weights flipped and transposed).
+//   k_dw_mfma    weight gradient: dW[t][co][ci] = sum_px dz[px][co] *
+//                a[px + off(t)][ci]; a workgroup stages one row of dz and the
+//                three rows of a it touches in LDS (72-byte pixel pitch, no
+//                bank conflicts for the transposed reads) and reduces over its
+//                rows with MFMAs whose K is 32 pixels; one extra MFMA against
+//                ones gives the bias gradient.  Per-workgroup partials are
+//                summed in a fixed order (deterministic).
+//   layer 6      (32 -> 1) and its gradients are VALU kernels: one output
+//                channel is no GEMM.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstddef>
+
+#include "lpbf_common.hpp"
+#include "lpbf_kernels.h"
+#include "lpbf_surrogate.h"
+
+namespace lpbf_amd {
+namespace {
+
+constexpr int kW = LPBF_SG_WIN;        // 128
+constexpr int kC = LPBF_SG_CH;         // 32
+constexpr int kPix = kW * kW;
+constexpr int kT = 256;
+constexpr int kPitch = 36;             // bf16 per pixel in LDS (72 bytes)
+constexpr int kDwSlots = 10;           // 9 taps + bias
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+// Parameter vector offsets (FP32 master layout).
+constexpr int kL1W = 9 * 4 * kC, kL1 = kL1W + kC;           // 1184
+constexpr int kLW = 9 * kC * kC, kLn = kLW + kC;            // 9248
+constexpr int kL6 = kL1 + 4 * kLn;                          // 38176
+static_assert(kL6 + 9 * kC + 1 == LPBF_SG_NPARAM, "parameter layout");
+
+// Pack layout (bytes).
+constexpr size_t kWsz = 9 * kC * kC;                        // bf16 per layer
+constexpr size_t kOffWF = 0;                                // [5][9][co][ci]
+constexpr size_t kOffWB = kOffWF + 5 * kWsz * 2;            // [5][9][out][in], flipped
+constexpr size_t kOffB = kOffWB + 5 * kWsz * 2;             // [5][32] fp32
+constexpr size_t kOffW6 = kOffB + 5 * kC * 4;               // [9][32] fp32 (bf16 values)
+constexpr size_t kOffB6 = kOffW6 + 9 * kC * 4;              // fp32
+constexpr size_t kPackBytes = (kOffB6 + 4 + 255) / 256 * 256;
+
+__device__ __host__ inline uint16_t f2bf(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h) {
+    return __builtin_bit_cast(float, static_cast<uint32_t>(h) << 16);
+}
+
+constexpr float kGeluC = 0.7978845608028654f;   // sqrt(2/pi)
+__device__ inline float gelu(float z) {
+    return 0.5f * z * (1.f + tanhf(kGeluC * (z + 0.044715f * z * z * z)));
+}
+__device__ inline float gelu_grad(float z) {
+    const float t = tanhf(kGeluC * (z + 0.044715f * z * z * z));
+    return 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * kGeluC * (1.f + 3.f * 0.044715f * z * z);
+}
+
+__device__ inline bf16x8 as_bf8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+// ---- pack ------------------------------------------------------------------
+
+__global__ void k_pack(const float* __restrict__ p, uint8_t* __restrict__ pack) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // over 5*9*32*32
+    uint16_t* wf = reinterpret_cast<uint16_t*>(pack + kOffWF);
+    uint16_t* wb = reinterpret_cast<uint16_t*>(pack + kOffWB);
+    if (i < 5 * static_cast<int>(kWsz)) {
+        const int l = i / kWsz, r = i % kWsz;
+        const int t = r / (kC * kC), co = (r / kC) % kC, ci = r % kC;
+        float w;
+        if (l == 0) w = ci < 4 ? p[(t * 4 + ci) * kC + co] : 0.f;
+        else w = p[kL1 + (l - 1) * kLn + (t * kC + ci) * kC + co];
+        wf[i] = f2bf(w);
+        // backward data of layer l: out channel = ci of the forward, in = co,
+        // tap flipped (the adjoint reads dz at p - off(t))
+        const int tb = 8 - t;
+        const int out = co, in = ci;   // reuse indices: entry [tb][out][in]
+        float wbv;
+        if (l == 0) wbv = out < 4 ? p[(t * 4 + out) * kC + in] : 0.f;
+        else wbv = p[kL1 + (l - 1) * kLn + (t * kC + out) * kC + in];
+        wb[static_cast<size_t>(l) * kWsz + (tb * kC + out) * kC + in] = f2bf(wbv);
+    }
+    if (i < 5 * kC) {
+        const int l = i / kC, c = i % kC;
+        reinterpret_cast<float*>(pack + kOffB)[i] =
+            l == 0 ? p[kL1W + c] : p[kL1 + (l - 1) * kLn + kLW + c];
+    }
+    if (i < 9 * kC)
+        reinterpret_cast<float*>(pack + kOffW6)[i] = bf2f(f2bf(p[kL6 + i]));
+    if (i == 0) reinterpret_cast<float*>(pack + kOffB6)[0] = p[kL6 + 9 * kC];
+}
+
+// ---- conv 32 -> 32 on MFMA --------------------------------------------------
+
+enum { kModeFwd = 0, kModeBwd = 1 };
+
+// X, out0, out1, zprev: [batch][128][128][32] bf16.  W: [9][32 out][32 in].
+template <int MODE>
+__global__ __launch_bounds__(kT) void k_conv_mfma(const uint16_t* __restrict__ X,
+                                                  const uint16_t* __restrict__ W,
+                                                  const float* __restrict__ bias,
+                                                  const uint16_t* __restrict__ zprev,
+                                                  uint16_t* __restrict__ out0,
+                                                  uint16_t* __restrict__ out1, int batch) {
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * (kT / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (gw >= batch * kW * 2) return;
+    const int img = gw / (2 * kW), row = (gw >> 1) % kW, x0 = (gw & 1) * 64;
+    const int lr = lane & 15, lg = lane >> 4;
+
+    bf16x8 w[9][2];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+            w[t][ct] = as_bf8(*reinterpret_cast<const uint4*>(
+                W + (static_cast<size_t>(t) * kC + ct * 16 + lr) * kC + 8 * lg));
+
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int pt = 0; pt < 4; ++pt) acc[ct][pt] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const uint16_t* Ximg = X + static_cast<size_t>(img) * kPix * kC;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int yy = row + t / 3 - 1;
+        if (yy < 0 || yy >= kW) continue;            // uniform: the whole row is padding
+        bf16x8 xv[4];
+#pragma unroll
+        for (int pt = 0; pt < 4; ++pt) {
+            const int xx = x0 + pt * 16 + lr + t % 3 - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (xx >= 0 && xx < kW)
+                v = *reinterpret_cast<const uint4*>(Ximg + (static_cast<size_t>(yy) * kW + xx) * kC +
+                                                    8 * lg);
+            xv[pt] = as_bf8(v);
+        }
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int pt = 0; pt < 4; ++pt)
+                acc[ct][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[t][ct], xv[pt], acc[ct][pt],
+                                                                      0, 0, 0);
+    }
+
+    // lane holds channels co0..co0+3 of pixel x0 + pt*16 + lr
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+        const int co0 = ct * 16 + 4 * lg;
+#pragma unroll
+        for (int pt = 0; pt < 4; ++pt) {
+            const size_t off = (static_cast<size_t>(img) * kPix + row * kW + x0 + pt * 16 + lr) * kC + co0;
+            uint16_t o0[4], o1[4];
+            if (MODE == kModeFwd) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const float z = acc[ct][pt][i] + bias[co0 + i];
+                    o0[i] = f2bf(z);
+                    o1[i] = f2bf(gelu(z));
+                }
+                if (out0)
+                    *reinterpret_cast<uint2*>(out0 + off) =
+                        make_uint2(o0[0] | (uint32_t(o0[1]) << 16), o0[2] | (uint32_t(o0[3]) << 16));
+                *reinterpret_cast<uint2*>(out1 + off) =
+                    make_uint2(o1[0] | (uint32_t(o1[1]) << 16), o1[2] | (uint32_t(o1[3]) << 16));
+            } else {
+                const uint2 zp = *reinterpret_cast<const uint2*>(zprev + off);
+                const uint16_t zs[4] = {uint16_t(zp.x & 0xffff), uint16_t(zp.x >> 16),
+                                        uint16_t(zp.y & 0xffff), uint16_t(zp.y >> 16)};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) o0[i] = f2bf(acc[ct][pt][i] * gelu_grad(bf2f(zs[i])));
+                *reinterpret_cast<uint2*>(out0 + off) =
+                    make_uint2(o0[0] | (uint32_t(o0[1]) << 16), o0[2] | (uint32_t(o0[3]) << 16));
+            }
+        }
+    }
+}
+
+// ---- weight gradient on MFMA ------------------------------------------------
+
+// partial[wg][slot][co][ci], slot 0..8 = taps, 9 = bias (ci column 0..15 of
+// the it == 0 waves; every column holds the same sum).
+__global__ __launch_bounds__(kT) void k_dw_mfma(const uint16_t* __restrict__ dz,
+                                                const uint16_t* __restrict__ A,
+                                                float* __restrict__ partial, int rows,
+                                                int rows_per_wg) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_dz[kW * kPitch];
+    __shared__ __attribute__((aligned(16))) uint16_t s_a[3 * (kW + 2) * kPitch];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ct = wave & 1, it = wave >> 1;
+    const int lr = lane & 15, lg = lane >> 4;
+    const int r0 = blockIdx.x * rows_per_wg;
+    const int r1 = min(r0 + rows_per_wg, rows);
+
+    f32x4 acc[kDwSlots];
+#pragma unroll
+    for (int s = 0; s < kDwSlots; ++s) acc[s] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t one = 0x3f80;
+    const uint4 ones4 = make_uint4(one | (uint32_t(one) << 16), one | (uint32_t(one) << 16),
+                                   one | (uint32_t(one) << 16), one | (uint32_t(one) << 16));
+    const bf16x8 ones = as_bf8(ones4);
+
+    for (int gr = r0; gr < r1; ++gr) {
+        const int img = gr / kW, y = gr % kW;
+        __syncthreads();
+        for (int q = tid; q < kW * 4; q += kT) {
+            const int px = q >> 2, c = q & 3;
+            const uint4 v = *reinterpret_cast<const uint4*>(
+                dz + (static_cast<size_t>(img) * kPix + y * kW + px) * kC + c * 8);
+            uint2* d = reinterpret_cast<uint2*>(s_dz + px * kPitch + c * 8);
+            d[0] = make_uint2(v.x, v.y);
+            d[1] = make_uint2(v.z, v.w);
+        }
+        for (int q = tid; q < 3 * (kW + 2) * 4; q += kT) {
+            const int rr = q / ((kW + 2) * 4), rem = q % ((kW + 2) * 4);
+            const int px = rem / 4 - 1, c = rem & 3;
+            const int yy = y + rr - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (yy >= 0 && yy < kW && px >= 0 && px < kW)
+                v = *reinterpret_cast<const uint4*>(
+                    A + (static_cast<size_t>(img) * kPix + yy * kW + px) * kC + c * 8);
+            uint2* d = reinterpret_cast<uint2*>(s_a + (rr * (kW + 2) + px + 1) * kPitch + c * 8);
+            d[0] = make_uint2(v.x, v.y);
+            d[1] = make_uint2(v.z, v.w);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kb = 0; kb < kW / 32; ++kb) {
+            bf16x8 a;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const uint16_t h = s_dz[(kb * 32 + 8 * lg + i) * kPitch + ct * 16 + lr];
+                a[i] = __builtin_bit_cast(__bf16, h);
+            }
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const int dy = t / 3, dx = t % 3;   // row in s_a, column shift + 1
+                bf16x8 b;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const uint16_t h =
+                        s_a[(dy * (kW + 2) + kb * 32 + 8 * lg + i + dx) * kPitch + it * 16 + lr];
+                    b[i] = __builtin_bit_cast(__bf16, h);
+                }
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+            }
+            if (it == 0) acc[9] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, ones, acc[9], 0, 0, 0);
+        }
+    }
+    float* P = partial + static_cast<size_t>(blockIdx.x) * kDwSlots * kC * kC;
+#pragma unroll
+    for (int s = 0; s < kDwSlots; ++s) {
+        if (s == 9 && it != 0) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            P[(s * kC + ct * 16 + 4 * lg + i) * kC + it * 16 + lr] = acc[s][i];
+    }
+}
+
+// Sum the per-workgroup partials (fixed order) and scatter into the FP32
+// gradient vector of layer l (0-based, 0..4).
+__global__ void k_dw_reduce(const float* __restrict__ partial, int nwg, int l,
+                            float* __restrict__ grad) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;   // slot*1024 + co*32 + ci
+    if (j >= kDwSlots * kC * kC) return;
+    const int s = j / (kC * kC), co = (j / kC) % kC, ci = j % kC;
+    if (s == 9 && ci != 0) return;
+    if (l == 0 && s < 9 && ci >= 4) return;
+    float sum = 0.f;
+    for (int w = 0; w < nwg; ++w) sum += partial[static_cast<size_t>(w) * kDwSlots * kC * kC + j];
+    if (l == 0) {
+        if (s < 9) grad[(s * 4 + ci) * kC + co] = sum;
+        else grad[kL1W + co] = sum;
+    } else {
+        float* g = grad + kL1 + (l - 1) * kLn;
+        if (s < 9) g[(s * kC + ci) * kC + co] = sum;
+        else g[kLW + co] = sum;
+    }
+}
+
+// ---- layer 6 (32 -> 1), VALU ------------------------------------------------
+
+// y = b6 + sum_t sum_ci a5[p + off(t)][ci] W6[t][ci]; with a target also the
+// loss gradient dy = 2 (y - target) / n_total and per-block sums of (y-t)^2.
+__global__ __launch_bounds__(kT) void k_out_fwd(const uint16_t* __restrict__ a5,
+                                                const uint8_t* __restrict__ pack,
+                                                const float* __restrict__ target, float* __restrict__ y,
+                                                float* __restrict__ dy, double* __restrict__ lpart,
+                                                int batch) {
+    const float* W6 = reinterpret_cast<const float*>(pack + kOffW6);
+    const float b6 = reinterpret_cast<const float*>(pack + kOffB6)[0];
+    const long long n = static_cast<long long>(batch) * kPix;
+    const long long p = static_cast<long long>(blockIdx.x) * kT + threadIdx.x;
+    double d2 = 0.0;
+    if (p < n) {
+        const int img = static_cast<int>(p / kPix), r = static_cast<int>(p % kPix);
+        const int py = r / kW, px = r % kW;
+        float acc = b6;
+        for (int t = 0; t < 9; ++t) {
+            const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
+            if (yy < 0 || yy >= kW || xx < 0 || xx >= kW) continue;
+            const uint4* src = reinterpret_cast<const uint4*>(
+                a5 + (static_cast<size_t>(img) * kPix + yy * kW + xx) * kC);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint4 v = src[c];
+                const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    acc = fmaf(bf2f(uint16_t(u[e] & 0xffff)), W6[t * kC + c * 8 + 2 * e], acc);
+                    acc = fmaf(bf2f(uint16_t(u[e] >> 16)), W6[t * kC + c * 8 + 2 * e + 1], acc);
+                }
+            }
+        }
+        y[p] = acc;
+        if (target) {
+            const float d = acc - target[p];
+            dy[p] = 2.f * d / static_cast<float>(n);
+            d2 = static_cast<double>(d) * d;
+        }
+    }
+    if (target) {
+        __shared__ double s[kT / 64];
+        d2 = wave_sum(d2);
+        if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = d2;
+        __syncthreads();
+        if (threadIdx.x == 0) lpart[blockIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
+    }
+}
+
+__global__ void k_loss_final(const double* __restrict__ lpart, int nb, long long n,
+                             double* __restrict__ loss) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < nb; ++i) s += lpart[i];
+    *loss = s / static_cast<double>(n);
+}
+
+// dz5[p][ci] = bf16( (sum_t dy[p - off(t)] W6[t][ci]) * gelu'(z5[p][ci]) )
+__global__ __launch_bounds__(kT) void k_out_bwd_data(const float* __restrict__ dy,
+                                                     const uint8_t* __restrict__ pack,
+                                                     const uint16_t* __restrict__ z5,
+                                                     uint16_t* __restrict__ dz5, int batch) {
+    const float* W6 = reinterpret_cast<const float*>(pack + kOffW6);
+    const long long n = static_cast<long long>(batch) * kPix;
+    const long long p = static_cast<long long>(blockIdx.x) * kT + threadIdx.x;
+    if (p >= n) return;
+    const int img = static_cast<int>(p / kPix), r = static_cast<int>(p % kPix);
+    const int py = r / kW, px = r % kW;
+    float g[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int yy = py - (t / 3 - 1), xx = px - (t % 3 - 1);
+        g[t] = (yy >= 0 && yy < kW && xx >= 0 && xx < kW)
+                   ? dy[static_cast<size_t>(img) * kPix + yy * kW + xx] : 0.f;
+    }
+    const uint4* zs = reinterpret_cast<const uint4*>(z5 + static_cast<size_t>(p) * kC);
+    uint4* out = reinterpret_cast<uint4*>(dz5 + static_cast<size_t>(p) * kC);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const uint4 zv = zs[c];
+        const uint32_t zu[4] = {zv.x, zv.y, zv.z, zv.w};
+        uint32_t ou[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            uint16_t h[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int ci = c * 8 + 2 * e + k;
+                float da = 0.f;
+#pragma unroll
+                for (int t = 0; t < 9; ++t) da = fmaf(g[t], W6[t * kC + ci], da);
+                const uint16_t zb = k ? uint16_t(zu[e] >> 16) : uint16_t(zu[e] & 0xffff);
+                h[k] = f2bf(da * gelu_grad(bf2f(zb)));
+            }
+            ou[e] = h[0] | (uint32_t(h[1]) << 16);
+        }
+        out[c] = make_uint4(ou[0], ou[1], ou[2], ou[3]);
+    }
+}
+
+// Per workgroup over its rows: dW6[t][ci] = sum a5[p+off(t)][ci] dy[p] and
+// db6 = sum dy (slot 288).  partial[wg][289].
+__global__ __launch_bounds__(kT) void k_out_dw(const uint16_t* __restrict__ a5,
+                                               const float* __restrict__ dy,
+                                               float* __restrict__ partial, int rows,
+                                               int rows_per_wg) {
+    __shared__ float s_dy[kW];
+    __shared__ __attribute__((aligned(16))) uint16_t s_a[3 * (kW + 2) * kPitch];
+    const int tid = threadIdx.x;
+    const int r0 = blockIdx.x * rows_per_wg, r1 = min(r0 + rows_per_wg, rows);
+    float acc0 = 0.f, acc1 = 0.f;   // outputs tid and tid + 256 (< 289)
+    for (int gr = r0; gr < r1; ++gr) {
+        const int img = gr / kW, y = gr % kW;
+        __syncthreads();
+        if (tid < kW) s_dy[tid] = dy[static_cast<size_t>(img) * kPix + y * kW + tid];
+        for (int q = tid; q < 3 * (kW + 2) * 4; q += kT) {
+            const int rr = q / ((kW + 2) * 4), rem = q % ((kW + 2) * 4);
+            const int px = rem / 4 - 1, c = rem & 3;
+            const int yy = y + rr - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (yy >= 0 && yy < kW && px >= 0 && px < kW)
+                v = *reinterpret_cast<const uint4*>(
+                    a5 + (static_cast<size_t>(img) * kPix + yy * kW + px) * kC + c * 8);
+            uint2* d = reinterpret_cast<uint2*>(s_a + (rr * (kW + 2) + px + 1) * kPitch + c * 8);
+            d[0] = make_uint2(v.x, v.y);
+            d[1] = make_uint2(v.z, v.w);
+        }
+        __syncthreads();
+        for (int k = 0; k < 2; ++k) {
+            const int o = tid + k * kT;
+            if (o > 9 * kC) continue;
+            float s = 0.f;
+            if (o == 9 * kC) {
+                for (int px = 0; px < kW; ++px) s += s_dy[px];
+            } else {
+                const int t = o / kC, ci = o % kC, dyr = t / 3, dx = t % 3;
+                for (int px = 0; px < kW; ++px)
+                    s = fmaf(bf2f(s_a[(dyr * (kW + 2) + px + dx) * kPitch + ci]), s_dy[px], s);
+            }
+            if (k == 0) acc0 += s; else acc1 += s;
+        }
+    }
+    float* P = partial + static_cast<size_t>(blockIdx.x) * (9 * kC + 1);
+    P[tid] = acc0;
+    if (tid + kT <= 9 * kC) P[tid + kT] = acc1;
+}
+
+__global__ void k_out_dw_reduce(const float* __restrict__ partial, int nwg,
+                                float* __restrict__ grad) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > 9 * kC) return;
+    float s = 0.f;
+    for (int w = 0; w < nwg; ++w) s += partial[static_cast<size_t>(w) * (9 * kC + 1) + j];
+    grad[kL6 + j] = s;
+}
+
+// ---- Adam ---------------------------------------------------------------------
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---- windows --------------------------------------------------------------------
+
+__global__ __launch_bounds__(kT) void k_gather(const lpbf_sg_window* __restrict__ win,
+                                               uint16_t* __restrict__ x0, float* __restrict__ T0) {
+    const int w = blockIdx.y;
+    const int p = blockIdx.x * kT + threadIdx.x;
+    if (p >= kPix) return;
+    const lpbf_sg_window& W = win[w];
+    const int i = p / kW, j = p % kW;
+    const int fx = W.ox + j, fy = W.oy + i;
+    // a window reaching outside its field reads zeros there
+    const bool in = fx >= 0 && fx < W.nx && fy >= 0 && fy < W.ny;
+    const float th = in ? W.theta[static_cast<size_t>(fy) * W.ld + fx] : 0.f;
+    T0[static_cast<size_t>(w) * kPix + p] = th;
+    const double xpos = fx * W.dx, ypos = fy * W.dy;
+    double src = 0.0;
+    for (int s = 0; s < LPBF_SG_STEPS; ++s) {
+        const LaserTick t = W.plan[W.step + s];
+        if (t.q_scale == 0.0) continue;
+        const double ddx = xpos - t.x_l, ddy = ypos - t.y_l;
+        src += t.q_scale * exp(-2.0 * (ddx * ddx + ddy * ddy) * W.inv_r0_sq);
+    }
+    src *= 1.0 / LPBF_SG_STEPS;
+    const uint16_t c0 = f2bf(th / 3000.f), c1 = f2bf(static_cast<float>(src));
+    const uint16_t c2 = f2bf(static_cast<float>(W.P / 400.0)), c3 = f2bf(static_cast<float>(W.V / 2.0));
+    uint4* out = reinterpret_cast<uint4*>(x0 + (static_cast<size_t>(w) * kPix + p) * kC);
+    out[0] = make_uint4(c0 | (uint32_t(c1) << 16), c2 | (uint32_t(c3) << 16), 0, 0);
+    out[1] = make_uint4(0, 0, 0, 0);
+    out[2] = make_uint4(0, 0, 0, 0);
+    out[3] = make_uint4(0, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(kT) void k_target(const lpbf_sg_window* __restrict__ win,
+                                               const float* __restrict__ T0,
+                                               float* __restrict__ tgt) {
+    const int w = blockIdx.y;
+    const int p = blockIdx.x * kT + threadIdx.x;
+    if (p >= kPix) return;
+    const lpbf_sg_window& W = win[w];
+    const int i = p / kW, j = p % kW;
+    const int fx = W.ox + j, fy = W.oy + i;
+    const bool in = fx >= 0 && fx < W.nx && fy >= 0 && fy < W.ny;
+    const float th = in ? W.theta[static_cast<size_t>(fy) * W.ld + fx] : 0.f;
+    const size_t q = static_cast<size_t>(w) * kPix + p;
+    tgt[q] = (th - T0[q]) / 3000.f;
+}
+
+__global__ __launch_bounds__(kT) void k_apply(const lpbf_sg_window* __restrict__ win,
+                                              const float* __restrict__ T0,
+                                              const float* __restrict__ y, int overlap,
+                                              const int* __restrict__ skip,
+                                              float* __restrict__ maxdiff) {
+    const int w = blockIdx.y;
+    const int p = blockIdx.x * kT + threadIdx.x;
+    if (p >= kPix) return;
+    const lpbf_sg_window& W = win[w];
+    const int i = p / kW, j = p % kW;
+    const int fx = W.ox + j, fy = W.oy + i;
+    const bool in = fx >= 0 && fx < W.nx && fy >= 0 && fy < W.ny;
+    const size_t q = static_cast<size_t>(w) * kPix + p;
+    float* cell = W.theta_out + (in ? static_cast<size_t>(fy) * W.ld + fx : 0);
+    const float pred = T0[q] + 3000.f * y[q];
+    const float cur = in ? *cell : pred;
+    if (maxdiff) {
+        float d = fabsf(pred - cur);
+        d = wave_max_f(d);
+        if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<int*>(maxdiff + w), __float_as_int(d));
+    }
+    if (!in || (skip && skip[w])) return;
+    const int e = min(min(i, j), min(kW - 1 - i, kW - 1 - j));
+    const float a = overlap > 0 ? fminf(1.f, static_cast<float>(e) / overlap) : 1.f;
+    *cell = a * pred + (1.f - a) * cur;
+}
+
+}  // namespace
+}  // namespace lpbf_amd
+
+using namespace lpbf_amd;
+
+namespace {
+
+// Workspace: z1..z5 (act[0..4], training only), a1..a5 (act[5..9]), and for
+// training two dz buffers, dy, the prediction, loss and gradient partials.
+struct Offsets {
+    size_t act[10], dz[2], dy, ypred, lpart, part, bytes;
+};
+
+constexpr size_t kAct = static_cast<size_t>(kPix) * kC * 2;   // bytes per window per tensor
+
+int dw_nwg(int rows) { return rows < 512 ? rows : 512; }
+
+Offsets offsets(int batch, bool training) {
+    Offsets o{};
+    size_t p = 0;
+    const size_t A = kAct * batch;
+    for (int i = 0; i < 10; ++i) {
+        if (i < 5 && !training) { o.act[i] = SIZE_MAX; continue; }
+        o.act[i] = p;
+        p += A;
+    }
+    if (training) {
+        o.dz[0] = p; p += A;
+        o.dz[1] = p; p += A;
+        o.dy = p; p += static_cast<size_t>(batch) * kPix * 4;
+        o.ypred = p; p += static_cast<size_t>(batch) * kPix * 4;
+        const size_t nb = (static_cast<size_t>(batch) * kPix + kT - 1) / kT;
+        o.lpart = p; p += (nb * 8 + 255) / 256 * 256;
+        o.part = p; p += static_cast<size_t>(dw_nwg(batch * kW)) * kDwSlots * kC * kC * 4;
+    }
+    o.bytes = p;
+    return o;
+}
+
+struct Work {
+    uint16_t* act[10];
+    uint16_t* dz[2];
+    float* dy;
+    float* ypred;
+    double* lpart;
+    float* part;
+};
+
+Work layout(void* base, int batch, bool training) {
+    const Offsets o = offsets(batch, training);
+    uint8_t* b = static_cast<uint8_t*>(base);
+    Work w{};
+    for (int i = 0; i < 10; ++i)
+        w.act[i] = o.act[i] == SIZE_MAX ? nullptr : reinterpret_cast<uint16_t*>(b + o.act[i]);
+    if (training) {
+        w.dz[0] = reinterpret_cast<uint16_t*>(b + o.dz[0]);
+        w.dz[1] = reinterpret_cast<uint16_t*>(b + o.dz[1]);
+        w.dy = reinterpret_cast<float*>(b + o.dy);
+        w.ypred = reinterpret_cast<float*>(b + o.ypred);
+        w.lpart = reinterpret_cast<double*>(b + o.lpart);
+        w.part = reinterpret_cast<float*>(b + o.part);
+    }
+    return w;
+}
+
+bool batch_ok(int b) { return b >= 1 && b <= 4096; }
+
+void conv_fwd(const uint16_t* X, const uint8_t* pack, int l, uint16_t* z, uint16_t* a, int batch,
+              hipStream_t s) {
+    const int waves = batch * kW * 2;
+    hipLaunchKernelGGL(k_conv_mfma<kModeFwd>, dim3((waves + 3) / 4), dim3(kT), 0, s, X,
+                       reinterpret_cast<const uint16_t*>(pack + kOffWF) + static_cast<size_t>(l) * kWsz,
+                       reinterpret_cast<const float*>(pack + kOffB) + l * kC,
+                       static_cast<const uint16_t*>(nullptr), z, a, batch);
+}
+
+void forward_impl(const uint8_t* pack, const uint16_t* x0, int batch, Work const& w, hipStream_t s) {
+    const uint16_t* in = x0;
+    for (int l = 0; l < 5; ++l) {
+        conv_fwd(in, pack, l, w.act[l], w.act[5 + l], batch, s);
+        in = w.act[5 + l];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t lpbf_sg_pack_bytes(void) { return kPackBytes; }
+
+size_t lpbf_sg_work_bytes(int batch, int training) {
+    if (!batch_ok(batch)) return 0;
+    // sizes only: lay out from a null base
+    return offsets(batch, training != 0).bytes;
+}
+
+void lpbf_sg_pack(const float* d_params, void* d_pack, hipStream_t stream) {
+    if (!d_params || !d_pack) { record_error(LPBF_ERR_INVALID_ARGUMENT); return; }
+    const int n = 5 * static_cast<int>(kWsz);
+    hipLaunchKernelGGL(k_pack, dim3((n + kT - 1) / kT), dim3(kT), 0, stream, d_params,
+                       static_cast<uint8_t*>(d_pack));
+    check_launch();
+}
+
+void lpbf_sg_forward(const void* d_pack, const void* d_x0, int batch, void* d_work, float* d_y,
+                     hipStream_t stream) {
+    if (!d_pack || !d_x0 || !d_work || !d_y || !batch_ok(batch)) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    const Work w = layout(d_work, batch, false);
+    const uint8_t* pack = static_cast<const uint8_t*>(d_pack);
+    forward_impl(pack, static_cast<const uint16_t*>(d_x0), batch, w, stream);
+    const long long n = static_cast<long long>(batch) * kPix;
+    hipLaunchKernelGGL(k_out_fwd, dim3(static_cast<int>((n + kT - 1) / kT)), dim3(kT), 0, stream,
+                       w.act[9], pack, static_cast<const float*>(nullptr), d_y,
+                       static_cast<float*>(nullptr), static_cast<double*>(nullptr), batch);
+    check_launch();
+}
+
+void lpbf_sg_loss_grad(const void* d_pack, const void* d_x0, const float* d_target, int batch,
+                       void* d_work, float* d_grad, double* d_loss, float* d_y,
+                       hipStream_t stream) {
+    if (!d_pack || !d_x0 || !d_target || !d_work || !d_grad || !d_loss || !batch_ok(batch)) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    const Work w = layout(d_work, batch, true);
+    const uint8_t* pack = static_cast<const uint8_t*>(d_pack);
+    const uint16_t* x0 = static_cast<const uint16_t*>(d_x0);
+    forward_impl(pack, x0, batch, w, stream);
+    const long long n = static_cast<long long>(batch) * kPix;
+    const int nb = static_cast<int>((n + kT - 1) / kT);
+    float* y = d_y ? d_y : w.ypred;
+    hipLaunchKernelGGL(k_out_fwd, dim3(nb), dim3(kT), 0, stream, w.act[9], pack, d_target, y,
+                       w.dy, w.lpart, batch);
+    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(64), 0, stream, w.lpart, nb, n, d_loss);
+    const int rows = batch * kW;
+    const int nwg = dw_nwg(rows), per = (rows + nwg - 1) / nwg;
+    // layer 6: weight gradient and dz5
+    hipLaunchKernelGGL(k_out_dw, dim3(nwg), dim3(kT), 0, stream, w.act[9], w.dy, w.part, rows, per);
+    hipLaunchKernelGGL(k_out_dw_reduce, dim3((9 * kC + 1 + 63) / 64), dim3(64), 0, stream, w.part,
+                       nwg, d_grad);
+    hipLaunchKernelGGL(k_out_bwd_data, dim3(nb), dim3(kT), 0, stream, w.dy, pack, w.act[4],
+                       w.dz[0], batch);
+    // layers 5..1
+    int cur = 0;
+    const int waves = batch * kW * 2;
+    for (int l = 4; l >= 0; --l) {
+        const uint16_t* a_prev = l > 0 ? w.act[5 + l - 1] : x0;
+        hipLaunchKernelGGL(k_dw_mfma, dim3(nwg), dim3(kT), 0, stream, w.dz[cur], a_prev, w.part,
+                           rows, per);
+        hipLaunchKernelGGL(k_dw_reduce, dim3((kDwSlots * kC * kC + kT - 1) / kT), dim3(kT), 0,
+                           stream, w.part, nwg, l, d_grad);
+        if (l > 0) {
+            hipLaunchKernelGGL(k_conv_mfma<kModeBwd>, dim3((waves + 3) / 4), dim3(kT), 0, stream,
+                               w.dz[cur],
+                               reinterpret_cast<const uint16_t*>(pack + kOffWB) +
+                                   static_cast<size_t>(l) * kWsz,
+                               static_cast<const float*>(nullptr), w.act[l - 1], w.dz[cur ^ 1],
+                               static_cast<uint16_t*>(nullptr), batch);
+            cur ^= 1;
+        }
+    }
+    check_launch();
+}
+
+void lpbf_sg_adam(float* d_params, const float* d_grad, float* d_m, float* d_v, int n, int t,
+                  float lr, float beta1, float beta2, float eps, hipStream_t stream) {
+    if (!d_params || !d_grad || !d_m || !d_v || n < 1 || t < 1) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    const float c1 = 1.f - powf(beta1, static_cast<float>(t));
+    const float c2 = 1.f - powf(beta2, static_cast<float>(t));
+    hipLaunchKernelGGL(k_adam, dim3((n + kT - 1) / kT), dim3(kT), 0, stream, d_params, d_grad, d_m,
+                       d_v, n, lr, beta1, beta2, eps, c1, c2);
+    check_launch();
+}
+
+void lpbf_sg_gather(const lpbf_sg_window* d_win, int n, void* d_x0, float* d_T0,
+                    hipStream_t stream) {
+    if (!d_win || n < 1 || n > 65535 || !d_x0 || !d_T0) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    hipLaunchKernelGGL(k_gather, dim3(kPix / kT, n), dim3(kT), 0, stream, d_win,
+                       static_cast<uint16_t*>(d_x0), d_T0);
+    check_launch();
+}
+
+void lpbf_sg_target(const lpbf_sg_window* d_win, int n, const float* d_T0, float* d_target,
+                    hipStream_t stream) {
+    if (!d_win || n < 1 || n > 65535 || !d_T0 || !d_target) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    hipLaunchKernelGGL(k_target, dim3(kPix / kT, n), dim3(kT), 0, stream, d_win, d_T0, d_target);
+    check_launch();
+}
+
+void lpbf_sg_apply(const lpbf_sg_window* d_win, int n, const float* d_T0, const float* d_y,
+                   int overlap, const int* d_skip, float* d_maxdiff, hipStream_t stream) {
+    if (!d_win || n < 1 || n > 65535 || !d_T0 || !d_y || overlap < 0 || overlap > kW / 2) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    hipLaunchKernelGGL(k_apply, dim3(kPix / kT, n), dim3(kT), 0, stream, d_win, d_T0, d_y, overlap,
+                       d_skip, d_maxdiff);
+    check_launch();
+}
+
+}  // extern "C"
