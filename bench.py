@@ -239,6 +239,92 @@ def case0_bench(args):
         "roofline": None, "cpu_baseline": None}), flush=True)
 
 
+def surrogate_bench(args):
+    """Training of the melt-pool surrogate on FD-generated windows (BF16 MFMA
+    forward/backward kernels, Adam lr 1e-3, batch 256), its held-out error
+    against persistence, and the surrogate-accelerated run on a 4096^2 plate
+    (10-track raster, P 300 W, V 1 m/s, 40,000 steps) against the plain FP32
+    run, both measured against the FP64 parity path."""
+    import numpy as np
+    import torch
+
+    import hedgehog_lpbf_amd as h
+    from hedgehog_lpbf_amd import surrogate as sg
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    steps = args.steps if args.steps is not None else 40000
+    spec = sg.DataSpec(n_runs=64, nx=768, ny=768, steps=12288, seed=0)
+    xs, T0s, tg = sg.generate(spec, log=log)
+    N = xs.shape[0]
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(N)
+    n_test = max(N // 20, 256)
+    test_idx, train_idx = perm[:n_test], perm[n_test:]
+    tr = torch.as_tensor(np.sort(train_idx), device="cuda")
+    xs_tr, tg_tr = xs[tr], tg[tr]
+    model = sg.Surrogate(seed=0)
+    # MFMA throughput: events around batch-256 forwards
+    x0 = torch.empty((256, sg.WIN, sg.WIN, sg.CH), dtype=torch.bfloat16, device="cuda")
+    sg.expand(xs_tr, torch.arange(256, device="cuda"), x0)
+    for _ in range(2):
+        model.forward(x0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    reps = 10
+    for _ in range(reps):
+        model.forward(x0)
+    e1.record()
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / reps
+    mfma_flop = 5 * 256 * sg.WIN * sg.WIN * sg.CH * 9 * sg.CH * 2   # the five MFMA convs
+    useful_flop = 256 * 1.255e9
+    t0 = time.perf_counter()
+    iters = args.warmup if args.warmup is not None else 3000
+    losses = sg.train(model, xs_tr, tg_tr, iters=iters, batch=256, lr=1e-3, log=log,
+                      log_every=250, time_budget=240.0)
+    train_s = time.perf_counter() - t0
+    ev = sg.evaluate(model, xs, T0s, tg, test_idx, batch=64)
+    log(f"[bench] held-out: {ev}")
+    del xs, T0s, tg, xs_tr, tg_tr
+    torch.cuda.empty_cache()
+
+    rs = sg.RunSpec(steps=steps)
+    mat, pr, dom, k, plan = sg.run_setup(rs)
+    log("[bench] FP64 reference run")
+    T64 = sg.fp64_reference(k, plan, steps)
+    T_plain, plain = sg.accelerated_run(None, rs, log=log)
+    T_acc, acc = sg.accelerated_run(model, rs, log=log)
+    err_plain = float((T_plain - T64).abs().max())
+    err_acc = float((T_acc - T64).abs().max())
+    sim_plain = plain["sim_s"] / plain["wall_s"]
+    sim_acc = acc["sim_s"] / acc["wall_s"]
+    md = acc.pop("recheck_maxdiff_K")
+    out = {
+        "metric": "sim-s/wall-s, surrogate-accelerated run (4096^2, 40k steps)",
+        "value": sim_acc, "unit": "sim-s/wall-s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": acc["wall_s"] * 1e3 / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (network) / f32 (FD)",
+        "data": "synthetic",
+        "config": {
+            "workload": "surrogate trained on FD windows (64 runs, 768^2, P U[200,350] W, "
+                        "V U[0.8,1.6] m/s, pairs every 16 steps), then a 4096^2 plate, 10-track "
+                        "raster P 300 W V 1 m/s, 128^2 window, 16 steps per call, recheck "
+                        "every 8 calls, fallback at 2 K",
+            "pairs": int(N), "train_iters": losses[-1][0] if losses else 0, "train_s": train_s,
+            "final_loss": losses[-1][1] if losses else None, "heldout": ev,
+            "plain_sim_s_per_wall_s": sim_plain, "speedup_vs_plain": sim_acc / sim_plain,
+            "max_abs_dT_vs_fp64_K": {"plain_fp32": err_plain, "accelerated": err_acc},
+            "run": acc, "recheck_maxdiff_K_first": md[:8],
+        },
+        "roofline": {"bound": "mfma", "kernel": "k_conv_mfma (5 layers, batch 256 forward)",
+                     "achieved": mfma_flop / (fwd_ms * 1e-3) / 1e12, "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": mfma_flop / (fwd_ms * 1e-3) / 1e12 / 2500.0,
+                     "traffic": None, "useful_TFLOPs": useful_flop / (fwd_ms * 1e-3) / 1e12,
+                     "forward_ms_batch256": fwd_ms},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def copy_reference(args):
     """Plain device-to-device copy of an n x n FP32 plate, `steps` times."""
     import torch
@@ -269,7 +355,8 @@ def main():
                          "20 repetitions for case0)")
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="plate8k_f32",
-                    choices=["plate8k_f32", "plate8k_f64", "batch256", "case0", "copy"],
+                    choices=["plate8k_f32", "plate8k_f64", "batch256", "case0", "surrogate",
+                             "copy"],
                     help="batch256: 256 uneven instances in one hipGraph launch; case0: the "
                          "reference's published case on the device-resident path; copy: a "
                          "plain device copy of one n x n FP32 plate per step, the reference "
@@ -296,6 +383,8 @@ def main():
         return batch_bench(args)
     if args.workload == "case0":
         return case0_bench(args)
+    if args.workload == "surrogate":
+        return surrogate_bench(args)
     if args.steps is None:
         args.steps = 4000 if args.workload != "copy" else 10
     if args.warmup is None:

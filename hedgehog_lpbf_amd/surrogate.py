@@ -179,7 +179,9 @@ def make_runs(spec: DataSpec):
         pr = Process(P=P, V=V, r0=spec.r0, h=50e-6, Tamb=300.0, Tsub=300.0)
         dom = Domain(Lx=(spec.nx - 1) * spec.dx, Ly=(spec.ny - 1) * spec.dx, nx=spec.nx,
                      ny=spec.ny)
-        rs = centred_raster(dom.Lx, dom.Ly, V, spec.dt, spec.steps, spec.n_tracks, spec.hatch)
+        # enough tracks that each stays within 70% of the plate width
+        n_tracks = max(spec.n_tracks, math.ceil(V * spec.dt * spec.steps / (0.7 * dom.Lx)))
+        rs = centred_raster(dom.Lx, dom.Ly, V, spec.dt, spec.steps, n_tracks, spec.hatch)
         plan = raster_plan(rs["x0"], rs["y0"], rs["length"], rs["hatch"], rs["n_tracks"],
                            rs["V"], spec.dt, spec.steps)
         runs.append((make_const(mat, pr, dom, spec.dt), plan, P, V))
@@ -303,3 +305,153 @@ def evaluate(model: Surrogate, xs, T0s, tg, idx, batch: int = 64):
         cnt += e_n.numel()
     return {"rms_net": math.sqrt(se_n / cnt), "max_net": mx_n,
             "rms_persistence": math.sqrt(se_p / cnt), "max_persistence": mx_p, "windows": cnt // WIN // WIN}
+
+
+# ---- surrogate-accelerated run ----------------------------------------------------------
+
+@dataclass
+class RunSpec:
+    nx: int = 4096
+    ny: int = 4096
+    dx: float = 5e-6
+    dt: float = 1.2e-6
+    steps: int = 40000
+    P: float = 300.0
+    V: float = 1.0
+    r0: float = 50e-6
+    n_tracks: int = 10
+    hatch: float = 80e-6
+    depth: int = 4
+    rows: int = 0
+    recheck_every: int = 8
+    tol_K: float = 2.0
+    overlap: int = 8
+
+
+def run_setup(rs: RunSpec):
+    from . import Domain, Process, centred_raster, make_const, raster_plan
+    mat = plate_material()
+    pr = Process(P=rs.P, V=rs.V, r0=rs.r0, h=50e-6, Tamb=300.0, Tsub=300.0)
+    dom = Domain(Lx=(rs.nx - 1) * rs.dx, Ly=(rs.ny - 1) * rs.dx, nx=rs.nx, ny=rs.ny)
+    r = centred_raster(dom.Lx, dom.Ly, rs.V, rs.dt, rs.steps, rs.n_tracks, rs.hatch)
+    plan = raster_plan(r["x0"], r["y0"], r["length"], r["hatch"], r["n_tracks"], r["V"],
+                       rs.dt, rs.steps + STEPS)
+    return mat, pr, dom, make_const(mat, pr, dom, rs.dt), plan
+
+
+class _F32Field:
+    """Two pitched FP32 theta buffers, the plan on the device and the step
+    workspace (theta = T - 300 K, field starts at 300 K)."""
+
+    def __init__(self, k, plan, depth, rows, device="cuda"):
+        import torch
+        self.k, self.depth, self.rows = k, depth, rows
+        self.ld = (k.nx + 3) // 4 * 4
+        self.a = torch.zeros((k.ny, self.ld), dtype=torch.float32, device=device)
+        self.b = torch.zeros_like(self.a)
+        self.plan = torch.from_numpy(np.ascontiguousarray(plan)).to(device)
+        self.work = torch.empty(lib().lpbf_f32_workspace_bytes(k.nx, k.ny, depth, rows) // 8 + 8,
+                                dtype=torch.float64, device=device)
+
+    def advance(self, s0, n, stream):
+        s = s0
+        while s < s0 + n:
+            d = min(self.depth, s0 + n - s)
+            lib().lpbf_launch_step_f32(self.a.data_ptr(), self.b.data_ptr(), self.ld, self.k,
+                                       300.0, 0, 300.0, 1609.0, self.plan.data_ptr(), s, d,
+                                       self.rows, None, self.work.data_ptr(), stream)
+            self.a, self.b = self.b, self.a
+            s += d
+
+    def T(self):
+        return self.a[:, :self.k.nx].double() + 300.0
+
+
+def fp64_reference(k, plan, steps, device="cuda"):
+    """The FP64 parity path (lpbf_launch_step_ex, mirror edges) over the run;
+    its step equals the CPU oracle's to 1e-9 K (tests/test_gpu_f64.py)."""
+    import torch
+
+    from ._lib import StepConst as SC
+    a = torch.full((k.ny, k.nx), 300.0, dtype=torch.float64, device=device)
+    b = torch.empty_like(a)
+    kk = SC.from_buffer_copy(bytes(k))
+    st = torch.cuda.current_stream().cuda_stream
+    for s in range(steps):
+        kk.x_l, kk.y_l, kk.q_scale = plan[s]
+        lib().lpbf_launch_step_ex(a.data_ptr(), b.data_ptr(), kk, 0, 0, 300.0, 1609.0, None, st)
+        a, b = b, a
+    torch.cuda.synchronize()
+    check()
+    return a
+
+
+def accelerated_run(model: Surrogate | None, rs: RunSpec, log=None):
+    """Plain FP32 FD over the whole plate, 16 steps at a time; with a model,
+    after every 16 steps the 128 x 128 window centred on the laser at the start
+    of the block is replaced by the surrogate's prediction from the field at
+    the start (blended over rs.overlap border cells).  Every rs.recheck_every
+    calls the prediction is compared with the FD result of the same window; a
+    difference above rs.tol_K keeps the FD result (fallback) and the next
+    rs.recheck_every - 1 calls are also left to FD.  Returns the final FP32
+    field (as T), wall seconds and counters."""
+    import torch
+    mat, pr, dom, k, plan = run_setup(rs)
+    f = _F32Field(k, plan, rs.depth, rs.rows)
+    stream = torch.cuda.current_stream().cuda_stream
+    x0 = torch.empty((1, WIN, WIN, CH), dtype=torch.bfloat16, device="cuda")
+    T0 = torch.empty((1, WIN, WIN), dtype=torch.float32, device="cuda")
+    y = torch.empty((1, WIN, WIN), dtype=torch.float32, device="cuda")
+    md = torch.zeros(1, dtype=torch.float32, device="cuda")
+    skip = torch.zeros(1, dtype=torch.int32, device="cuda")
+    calls = rechecks = fallbacks = applied = 0
+    off_until = -1
+    since_check = 0     # calls applied since the last recheck; 0: the next call is checked
+    diffs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n_blocks = rs.steps // STEPS
+    for blk in range(n_blocks):
+        s = blk * STEPS
+        use = model is not None and blk > off_until and plan[s][2] != 0.0
+        if use:
+            ox, oy = window_origin(plan[s][0], plan[s][1], k.dx, k.dy, k.nx, k.ny)
+            w = SgWindow(f.a.data_ptr(), 0, f.plan.data_ptr(), 300.0, k.dx, k.dy, k.inv_r0_sq,
+                         rs.P, rs.V, f.ld, k.nx, k.ny, ox, oy, s)
+            wd = windows_buffer([w])
+            gather(wd, 1, x0, T0)
+            model.forward(x0, out=y)
+        f.advance(s, STEPS, stream)
+        if use:
+            calls += 1
+            w.theta_out = f.a.data_ptr()
+            wd = windows_buffer([w])
+            if since_check == 0:
+                # compare with the FD result of the same window, write nothing
+                md.zero_()
+                skip.fill_(1)
+                apply(wd, 1, T0, y, rs.overlap, skip, md)
+                d = float(md.item())
+                diffs.append(d)
+                rechecks += 1
+                if d > rs.tol_K:
+                    # keep FD here and for the next recheck_every - 1 blocks;
+                    # the first call after that is checked again
+                    fallbacks += 1
+                    off_until = blk + rs.recheck_every - 1
+                    continue
+            apply(wd, 1, T0, y, rs.overlap, None, None)
+            applied += 1
+            since_check = (since_check + 1) % rs.recheck_every
+    f.advance(n_blocks * STEPS, rs.steps - n_blocks * STEPS, stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    check()
+    out = {"wall_s": wall, "sim_s": rs.steps * rs.dt, "calls": calls, "applied": applied,
+           "rechecks": rechecks, "fallbacks": fallbacks,
+           "fallback_rate": fallbacks / rechecks if rechecks else None,
+           "recheck_maxdiff_K": diffs}
+    if log:
+        log(f"[surrogate-run] {'model' if model is not None else 'plain'}: {wall:.2f} s wall, "
+            f"{calls} calls, {applied} applied, {fallbacks}/{rechecks} rechecks fell back")
+    return f.T(), out
