@@ -58,7 +58,8 @@ class RunOptions(C.Structure):
     """include/lpbf_run.h lpbf_run_options."""
     _fields_ = [("precision", C.c_int), ("bc", C.c_int), ("T_bc", C.c_double),
                 ("batch", C.c_int), ("depth", C.c_int), ("rows", C.c_int),
-                ("T_ref", C.c_double), ("keep_records", C.c_int)]
+                ("T_ref", C.c_double), ("keep_records", C.c_int), ("graph", C.c_int),
+                ("pad_", C.c_int)]
 
 
 class StepRecord(C.Structure):
@@ -102,6 +103,9 @@ SIGNATURES = {
     "lpbf_launch_step_ex": (_v, [_vp, _vp, _KP, _i, _i, _d, _d, _vp, _vp]),
     "lpbf_step_ex_nblocks": (_i, [_i, _i]),
     "lpbf_launch_stats_finalize": (_v, [_vp, _i, _i, _i, _vp, _vp]),
+    "lpbf_step_ex_plan_nblocks": (_i, [_i, _i]),
+    "lpbf_launch_step_ex_plan": (_v, [_vp, _vp, _KP, _i, _i, _d, _d, _vp, _vp, _vp, _vp, _vp,
+                                      _vp, _i, _vp, _vp]),
     "lpbf_launch_noise_f64": (_v, [_vp, _ll, _ull, _d, _d, _vp]),
     # FP32 path
     "lpbf_f32_workspace_bytes": (_sz, [_i, _i, _i, _i]),
@@ -147,6 +151,7 @@ SIGNATURES = {
     "lpbf_run_download": (_i, [_vp, _vp]),
     "lpbf_run_summary": (_i, [_vp, _DP, C.POINTER(C.c_int)]),
     "lpbf_run_records": (_i, [_vp, _i, _i, C.POINTER(StepRecord)]),
+    "lpbf_run_graph_launches": (_i, [_vp]),
     # host helpers (include/lpbf_host.h)
     "lpbf_make_const": (_v, [_DP, _DP, _i, _d, _d, _i, _i, _d, _KP]),
     "lpbf_dt_cfl": (_d, [_DP, _d, _d, _i, _i]),

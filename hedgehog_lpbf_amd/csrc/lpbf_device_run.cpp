@@ -84,6 +84,31 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
             if (nblk_ <= 0) throw std::invalid_argument("DeviceRun: grid too large");
             hip_ok(hipMalloc(&d_blk_, static_cast<std::size_t>(nblk_) * sizeof(StepStats)),
                    "block records");
+            buf0_ = d_a_;
+            if (opt_.graph && opt_.batch % 2 == 0 && !plan_.empty()) {
+                std::vector<LaserTick> lt = to_laser(plan_);
+                hip_ok(hipMalloc(reinterpret_cast<void**>(&d_plan_), lt.size() * sizeof(LaserTick)),
+                       "plan");
+                hip_ok(hipMemcpy(d_plan_, lt.data(), lt.size() * sizeof(LaserTick),
+                                 hipMemcpyHostToDevice), "plan upload");
+                std::vector<unsigned char> cr(plan_.size());
+                for (std::size_t i = 0; i < plan_.size(); ++i) cr[i] = cruise(plan_[i]) ? 1 : 0;
+                hip_ok(hipMalloc(reinterpret_cast<void**>(&d_cruise_), cr.size()), "cruise flags");
+                hip_ok(hipMemcpy(d_cruise_, cr.data(), cr.size(), hipMemcpyHostToDevice),
+                       "cruise upload");
+                hip_ok(hipMalloc(reinterpret_cast<void**>(&d_step_), 2 * sizeof(int)),
+                       "step counter and ticket");
+                hip_ok(hipMemset(d_step_, 0, 2 * sizeof(int)), "ticket");
+                const int pb = lpbf_step_ex_plan_nblocks(nx_, ny_);
+                if (pb > nblk_) {
+                    hip_ok(hipFree(d_blk_), "block records");
+                    d_blk_ = nullptr;
+                    hip_ok(hipMalloc(&d_blk_, static_cast<std::size_t>(pb) * sizeof(StepStats)),
+                           "block records");
+                }
+                hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_step_), sizeof(int)),
+                       "pinned step");
+            }
         } else {
             ld_ = (nx_ + 3) & ~3;
             const std::size_t F = static_cast<std::size_t>(ld_) * ny_ * sizeof(float);
@@ -121,6 +146,17 @@ DeviceRun::~DeviceRun() { release(); }
 // the next HIP call of the caller.
 void DeviceRun::release() noexcept {
     if (stream_) (void)hipStreamSynchronize(S(stream_));
+    for (int q = 0; q < 2; ++q) {
+        if (gexec_[q]) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(gexec_[q]));
+        if (graph_[q]) (void)hipGraphDestroy(static_cast<hipGraph_t>(graph_[q]));
+        gexec_[q] = graph_[q] = nullptr;
+    }
+    if (d_step_) (void)hipFree(d_step_);
+    if (d_cruise_) (void)hipFree(d_cruise_);
+    if (h_step_) (void)hipHostFree(h_step_);
+    d_step_ = nullptr;
+    d_cruise_ = nullptr;
+    h_step_ = nullptr;
     for (void* p : {d_a_, d_b_, d_blk_, static_cast<void*>(d_plan_), static_cast<void*>(d_ring_),
                     static_cast<void*>(d_cols_), static_cast<void*>(d_snap_)})
         if (p) (void)hipFree(p);
@@ -184,7 +220,9 @@ void DeviceRun::download(double* T) const {
 int DeviceRun::advance(int n) {
     int ran = 0;
     while (ran < n && done_ < static_cast<int>(plan_.size())) {
-        const int bn = std::min({opt_.batch, n - ran, static_cast<int>(plan_.size()) - done_});
+        // batches end at multiples of `batch`, so full ones can replay the graph
+        const int to_edge = opt_.batch - done_ % opt_.batch;
+        const int bn = std::min({to_edge, n - ran, static_cast<int>(plan_.size()) - done_});
         batch(bn);
         ran += bn;
     }
@@ -221,6 +259,52 @@ void DeviceRun::finish_step(Tick const& tk, StepStats const& st, const double* c
     if (opt_.keep_records) rec_.push_back(last_);
 }
 
+// One batch of opt_.batch steps (even) as kernel nodes, one per step: the
+// step with fused statistics, whose last workgroup merges them into
+// ring[s % batch], copies the hot column on cruise steps and advances the
+// device step counter.  `parity` names the physical buffer holding the
+// current field; an even count leaves it there.
+void DeviceRun::capture(int parity) {
+    const hipStream_t s = S(stream_);
+    const int B = opt_.batch;
+    double* cur = static_cast<double*>(parity == 0 ? buf0_ : (buf0_ == d_a_ ? d_b_ : d_a_));
+    double* nxt = static_cast<double*>(cur == d_a_ ? d_b_ : d_a_);
+    hipGraph_t g = nullptr;
+    hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+    for (int i = 0; i < B; ++i) {
+        lpbf_launch_step_ex_plan(cur, nxt, &k_, phys_, opt_.bc, opt_.T_bc, Tm_, d_plan_, d_step_,
+                                 d_cruise_, static_cast<StepStats*>(d_blk_), d_step_ + 1, d_ring_,
+                                 B, d_cols_, s);
+        std::swap(cur, nxt);
+    }
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        hip_ok(e, "end capture");
+    }
+    graph_[parity] = g;
+    lib_ok("captured launches");
+    hipGraphExec_t x = nullptr;
+    hip_ok(hipGraphInstantiateWithFlags(&x, g, 0), "instantiate");
+    gexec_[parity] = x;
+}
+
+bool DeviceRun::graph_batch(int s0, int bn) {
+    if (opt_.precision != 64 || !d_step_ || bn != opt_.batch || s0 % opt_.batch != 0) return false;
+    if (snap_cb_)
+        for (int i = 0; i < bn; ++i)
+            if (plan_[static_cast<std::size_t>(s0 + i)].dump) return false;
+    const int parity = (d_a_ == buf0_) ? 0 : 1;
+    if (!gexec_[parity]) capture(parity);
+    *h_step_ = s0;
+    hip_ok(hipMemcpyAsync(d_step_, h_step_, sizeof(int), hipMemcpyHostToDevice, S(stream_)),
+           "step counter");
+    hip_ok(hipGraphLaunch(static_cast<hipGraphExec_t>(gexec_[parity]), S(stream_)),
+           "graph launch");
+    ++graph_launches_;
+    return true;
+}
+
 void DeviceRun::batch(int bn) {
     const hipStream_t s = S(stream_);
     const std::size_t N = static_cast<std::size_t>(nx_) * ny_;
@@ -229,7 +313,10 @@ void DeviceRun::batch(int bn) {
     std::vector<char> has_col(static_cast<std::size_t>(bn), 0);
     const int s0 = done_;
 
-    if (opt_.precision == 64) {
+    if (graph_batch(s0, bn)) {
+        for (int i = 0; i < bn; ++i)
+            has_col[static_cast<std::size_t>(i)] = cruise(plan_[static_cast<std::size_t>(s0 + i)]);
+    } else if (opt_.precision == 64) {
         for (int i = 0; i < bn; ++i) {
             const Tick& tk = plan_[static_cast<std::size_t>(s0 + i)];
             k_.x_l = tk.x_l; k_.y_l = tk.y_l; k_.q_scale = tk.q_scale;
@@ -370,6 +457,7 @@ int lpbf_run_create(const StepConst* k, int phys, double Tm, double Lx, double L
             o.rows = opt->rows;
             o.T_ref = opt->T_ref;
             o.keep_records = opt->keep_records != 0;
+            o.graph = opt->graph != 0;
         }
         *out = new lpbf_run(dom, *k, phys, Tm, std::move(ticks), Tamb, o);
     });
@@ -409,6 +497,11 @@ int lpbf_run_summary(const lpbf_run* run, double* out3, int* steps_done) {
     out3[2] = run->run.steady_W_sub();
     if (steps_done) *steps_done = run->run.steps_done();
     return 0;
+}
+
+int lpbf_run_graph_launches(const lpbf_run* run) {
+    if (!run) { lpbf_amd::record_error(LPBF_ERR_INVALID_ARGUMENT); return -LPBF_ERR_INVALID_ARGUMENT; }
+    return run->run.graph_launches();
 }
 
 int lpbf_run_records(const lpbf_run* run, int first, int n, lpbf_step_record* out) {

@@ -127,7 +127,7 @@ __device__ __forceinline__ double cell_ext(double T, double Txm, double Txp,
 }
 
 enum { BC_MIRROR = LPBF_BC_MIRROR, BC_DIRICHLET = LPBF_BC_DIRICHLET, BC_TILE = 2 };
-enum { RED_NONE = 0, RED_PARTIAL = 1, RED_STATS = 2 };
+enum { RED_NONE = 0, RED_PARTIAL = 1, RED_STATS = 2, RED_RING = 3 };
 
 // The rows being updated.  For BC_TILE, in/out point at the first owned row
 // and rows -1 and ny (the halos) exist in both buffers.
@@ -143,11 +143,29 @@ struct Region {
     int nwx;        // wavefront blocks across
     int ntasks;     // wavefront blocks in total
     int ny_glob;    // global row count (reduction identity)
+    // plan-driven launches (graph replay): the laser of step *step comes from
+    // plan[*step] instead of k's x_l, y_l, q_scale; with RED_RING the last
+    // workgroup to finish (ticket) merges the block records into
+    // ring[s % ring_len], copies the hot column on cruise steps, resets the
+    // ticket and advances *step
+    const LaserTick* plan;
+    int* step;
+    int* ticket;
+    StepStats* ring;
+    int ring_len;
+    const unsigned char* cruise;
+    double* cols;
 };
 
 template <int PHYS, int RED>
 __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                                                        void* red_out, int red_fill) {
+    if (g.plan) {
+        const LaserTick t = g.plan[*g.step];
+        k.x_l = t.x_l;
+        k.y_l = t.y_l;
+        k.q_scale = t.q_scale;
+    }
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int task = blockIdx.x * (kThreads / kWave) + wave;
@@ -235,7 +253,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                         if (yg > pa.ymax) pa.ymax = yg;
                         ++molten;
                     }
-                    if (RED == RED_STATS && upd) {
+                    if ((RED == RED_STATS || RED == RED_RING) && upd) {
                         s_dT += v - T;
                         s_src += src;
                         s_loss += lossv;
@@ -254,7 +272,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
     const PartialAcc w = wave_reduce(pa);
     double w_dT = 0.0, w_src = 0.0, w_loss = 0.0;
     long long w_mol = 0;
-    if (RED == RED_STATS) {
+    if (RED == RED_STATS || RED == RED_RING) {
         w_dT = wave_sum(s_dT);
         w_src = wave_sum(s_src);
         w_loss = wave_sum(s_loss);
@@ -284,6 +302,34 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                 st.sum_loss += s_sum[2][i];
             }
             static_cast<StepStats*>(red_out)[blockIdx.x] = st;
+        }
+    }
+    if (RED == RED_RING) {
+        // last-block merge (no waiting: the workgroup holding the final ticket
+        // does the step's remaining work)
+        __shared__ int s_last, s_xhot;
+        if (threadIdx.x == 0) {
+            __threadfence();
+            s_last = atomicAdd(g.ticket, 1) == static_cast<int>(gridDim.x) - 1;
+        }
+        __syncthreads();
+        if (s_last) {
+            __threadfence();
+            const int s = *g.step;
+            StepStats* slot = g.ring + s % g.ring_len;
+            block_merge_stats(static_cast<const StepStats*>(red_out), gridDim.x, nx, g.ny_glob,
+                              slot);
+            if (threadIdx.x == 0) s_xhot = static_cast<int>(slot->arg % nx);
+            __syncthreads();
+            if (g.cols && g.cruise && g.cruise[s]) {
+                double* col = g.cols + static_cast<long long>(s % g.ring_len) * ny;
+                for (int y = threadIdx.x; y < ny; y += kThreads)
+                    col[y] = g.out[static_cast<long long>(y) * nx + s_xhot];
+            }
+            if (threadIdx.x == 0) {
+                *g.ticket = 0;
+                *g.step = s + 1;
+            }
         }
     }
     if (RED == RED_PARTIAL) {
@@ -381,6 +427,13 @@ Region make_region(const double* in, double* out, int nx, int ny, int y_off, int
     g.nwx = (nx + kBlockCols - 1) / kBlockCols;
     g.ntasks = g.nwx * ((ny + g.R - 1) / g.R);
     g.ny_glob = ny_glob;
+    g.plan = nullptr;
+    g.step = nullptr;
+    g.ticket = nullptr;
+    g.ring = nullptr;
+    g.ring_len = 1;
+    g.cruise = nullptr;
+    g.cols = nullptr;
     return g;
 }
 
@@ -407,6 +460,25 @@ void launch_step(const Region& g, const StepConst& k, int phys, void* red, int f
 }
 
 bool valid_grid(const StepConst* k) { return k && shape_ok(k->nx, k->ny); }
+
+// Band height of the plan-driven launch: halved from 32 rows until the grid
+// has 4096 wavefront tiles (or 2 rows), so small plates still spread over
+// the chip; a launch that is latency-bound gains more from parallel rows
+// than it loses to the extra halo rows.
+int plan_rows(int nx, int ny) {
+    const int nwx = (nx + kBlockCols - 1) / kBlockCols;
+    int R = kRowsPerWave;
+    while (R > 2 && static_cast<long long>(nwx) * ((ny + R - 1) / R) < 4096) R /= 2;
+    return R;
+}
+
+Region plan_region(const double* in, double* out, int nx, int ny, int bc, double T_bc,
+                   double Tm) {
+    Region g = make_region(in, out, nx, ny, 0, bc, T_bc, Tm, ny);
+    g.R = plan_rows(nx, ny);
+    g.ntasks = g.nwx * ((ny + g.R - 1) / g.R);
+    return g;
+}
 
 }  // namespace
 }  // namespace lpbf_amd
@@ -500,6 +572,33 @@ void lpbf_launch_step_ex(const double* Tin, double* Tout, const StepConst* k, in
         launch_step<RED_STATS>(g, *k, phys, block_stats, 0, stream);
     else
         launch_step<RED_NONE>(g, *k, phys, nullptr, 0, stream);
+}
+
+int lpbf_step_ex_plan_nblocks(int nx, int ny) {
+    if (!shape_ok(nx, ny)) return 0;
+    return grid_of(plan_region(nullptr, nullptr, nx, ny, BC_MIRROR, 0.0, 0.0));
+}
+
+void lpbf_launch_step_ex_plan(const double* Tin, double* Tout, const StepConst* k, int phys,
+                              int bc, double T_bc, double Tm, const LaserTick* d_plan,
+                              int* d_step, const unsigned char* d_cruise,
+                              StepStats* block_stats, int* d_ticket, StepStats* ring,
+                              int ring_len, double* cols, hipStream_t stream) {
+    if (!valid_grid(k) || !Tin || !Tout || Tin == Tout || !d_plan || !d_step || !block_stats ||
+        !d_ticket || !ring || ring_len < 1 || (phys != 0 && phys != 1) ||
+        (bc != LPBF_BC_MIRROR && bc != LPBF_BC_DIRICHLET)) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    Region g = plan_region(Tin, Tout, k->nx, k->ny, bc, T_bc, Tm);
+    g.plan = d_plan;
+    g.step = d_step;
+    g.ticket = d_ticket;
+    g.ring = ring;
+    g.ring_len = ring_len;
+    g.cruise = d_cruise;
+    g.cols = cols;
+    launch_step<RED_RING>(g, *k, phys, block_stats, 0, stream);
 }
 
 void lpbf_launch_stats_finalize(const StepStats* parts, int nblocks, int nx, int ny,

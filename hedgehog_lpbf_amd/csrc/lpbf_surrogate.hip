@@ -288,8 +288,18 @@ __global__ void k_dw_reduce(const float* __restrict__ partial, int nwg, int l,
     const int s = j / (kC * kC), co = (j / kC) % kC, ci = j % kC;
     if (s == 9 && ci != 0) return;
     if (l == 0 && s < 9 && ci >= 4) return;
-    float sum = 0.f;
-    for (int w = 0; w < nwg; ++w) sum += partial[static_cast<size_t>(w) * kDwSlots * kC * kC + j];
+    // four independent chains (fixed order: deterministic) hide the load latency
+    const size_t stride = static_cast<size_t>(kDwSlots) * kC * kC;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int w = 0;
+    for (; w + 4 <= nwg; w += 4) {
+        s0 += partial[(w + 0) * stride + j];
+        s1 += partial[(w + 1) * stride + j];
+        s2 += partial[(w + 2) * stride + j];
+        s3 += partial[(w + 3) * stride + j];
+    }
+    for (; w < nwg; ++w) s0 += partial[w * stride + j];
+    const float sum = (s0 + s1) + (s2 + s3);
     if (l == 0) {
         if (s < 9) grad[(s * 4 + ci) * kC + co] = sum;
         else grad[kL1W + co] = sum;
@@ -350,12 +360,19 @@ __global__ __launch_bounds__(kT) void k_out_fwd(const uint16_t* __restrict__ a5,
     }
 }
 
-__global__ void k_loss_final(const double* __restrict__ lpart, int nb, long long n,
-                             double* __restrict__ loss) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double s = 0.0;
-    for (int i = 0; i < nb; ++i) s += lpart[i];
-    *loss = s / static_cast<double>(n);
+// One workgroup: strided per-thread sums, then a fixed tree (deterministic).
+__global__ __launch_bounds__(kT) void k_loss_final(const double* __restrict__ lpart, int nb,
+                                                   long long n, double* __restrict__ loss) {
+    __shared__ double s[kT];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nb; i += kT) a += lpart[i];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int h = kT / 2; h > 0; h >>= 1) {
+        if (static_cast<int>(threadIdx.x) < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss = s[0] / static_cast<double>(n);
 }
 
 // dz5[p][ci] = bf16( (sum_t dy[p - off(t)] W6[t][ci]) * gelu'(z5[p][ci]) )
@@ -402,7 +419,9 @@ __global__ __launch_bounds__(kT) void k_out_bwd_data(const float* __restrict__ d
 }
 
 // Per workgroup over its rows: dW6[t][ci] = sum a5[p+off(t)][ci] dy[p] and
-// db6 = sum dy (slot 288).  partial[wg][289].
+// db6 = sum dy (slot 288).  partial[wg][289].  Thread (r, ci), r = tap row,
+// walks the staged a5 row once and feeds the three taps of that row from a
+// sliding window of dy (one LDS read of a and one of dy per column).
 __global__ __launch_bounds__(kT) void k_out_dw(const uint16_t* __restrict__ a5,
                                                const float* __restrict__ dy,
                                                float* __restrict__ partial, int rows,
@@ -411,7 +430,9 @@ __global__ __launch_bounds__(kT) void k_out_dw(const uint16_t* __restrict__ a5,
     __shared__ __attribute__((aligned(16))) uint16_t s_a[3 * (kW + 2) * kPitch];
     const int tid = threadIdx.x;
     const int r0 = blockIdx.x * rows_per_wg, r1 = min(r0 + rows_per_wg, rows);
-    float acc0 = 0.f, acc1 = 0.f;   // outputs tid and tid + 256 (< 289)
+    const int tr = tid / kC, ci = tid % kC;          // tap row 0..2 for tid < 96
+    float acc[3] = {0.f, 0.f, 0.f};
+    float accb = 0.f;
     for (int gr = r0; gr < r1; ++gr) {
         const int img = gr / kW, y = gr % kW;
         __syncthreads();
@@ -429,32 +450,45 @@ __global__ __launch_bounds__(kT) void k_out_dw(const uint16_t* __restrict__ a5,
             d[1] = make_uint2(v.z, v.w);
         }
         __syncthreads();
-        for (int k = 0; k < 2; ++k) {
-            const int o = tid + k * kT;
-            if (o > 9 * kC) continue;
-            float s = 0.f;
-            if (o == 9 * kC) {
-                for (int px = 0; px < kW; ++px) s += s_dy[px];
-            } else {
-                const int t = o / kC, ci = o % kC, dyr = t / 3, dx = t % 3;
-                for (int px = 0; px < kW; ++px)
-                    s = fmaf(bf2f(s_a[(dyr * (kW + 2) + px + dx) * kPitch + ci]), s_dy[px], s);
+        if (tid < 3 * kC) {
+            float d1 = 0.f, d2 = 0.f;                  // dy[c-1], dy[c-2]
+            for (int c = 0; c < kW + 2; ++c) {
+                const float d0 = c < kW ? s_dy[c] : 0.f;
+                const float a = bf2f(s_a[(tr * (kW + 2) + c) * kPitch + ci]);
+                acc[0] = fmaf(a, d0, acc[0]);           // dx = 0: pixel c
+                acc[1] = fmaf(a, d1, acc[1]);           // dx = 1: pixel c - 1
+                acc[2] = fmaf(a, d2, acc[2]);           // dx = 2: pixel c - 2
+                d2 = d1;
+                d1 = d0;
             }
-            if (k == 0) acc0 += s; else acc1 += s;
+        } else if (tid == 3 * kC) {
+            for (int px = 0; px < kW; ++px) accb += s_dy[px];
         }
     }
     float* P = partial + static_cast<size_t>(blockIdx.x) * (9 * kC + 1);
-    P[tid] = acc0;
-    if (tid + kT <= 9 * kC) P[tid + kT] = acc1;
+    if (tid < 3 * kC) {
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) P[(tr * 3 + dx) * kC + ci] = acc[dx];
+    } else if (tid == 3 * kC) {
+        P[9 * kC] = accb;
+    }
 }
 
 __global__ void k_out_dw_reduce(const float* __restrict__ partial, int nwg,
                                 float* __restrict__ grad) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j > 9 * kC) return;
-    float s = 0.f;
-    for (int w = 0; w < nwg; ++w) s += partial[static_cast<size_t>(w) * (9 * kC + 1) + j];
-    grad[kL6 + j] = s;
+    const size_t stride = 9 * kC + 1;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int w = 0;
+    for (; w + 4 <= nwg; w += 4) {
+        s0 += partial[(w + 0) * stride + j];
+        s1 += partial[(w + 1) * stride + j];
+        s2 += partial[(w + 2) * stride + j];
+        s3 += partial[(w + 3) * stride + j];
+    }
+    for (; w < nwg; ++w) s0 += partial[w * stride + j];
+    grad[kL6 + j] = (s0 + s1) + (s2 + s3);
 }
 
 // ---- Adam ---------------------------------------------------------------------
@@ -682,7 +716,7 @@ void lpbf_sg_loss_grad(const void* d_pack, const void* d_x0, const float* d_targ
     float* y = d_y ? d_y : w.ypred;
     hipLaunchKernelGGL(k_out_fwd, dim3(nb), dim3(kT), 0, stream, w.act[9], pack, d_target, y,
                        w.dy, w.lpart, batch);
-    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(64), 0, stream, w.lpart, nb, n, d_loss);
+    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(kT), 0, stream, w.lpart, nb, n, d_loss);
     const int rows = batch * kW;
     const int nwg = dw_nwg(rows), per = (rows + nwg - 1) / nwg;
     // layer 6: weight gradient and dz5

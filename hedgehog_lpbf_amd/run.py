@@ -26,7 +26,7 @@ class DeviceRun:
                  plan: np.ndarray, x_csv=None, wrapped=None, dump=None, phys: int = 0,
                  precision: int = 64, bc: int = 0, T_bc: float = 300.0, batch: int = 64,
                  depth: int = 4, rows: int = 0, T_ref: float = -1.0,
-                 keep_records: bool = True):
+                 keep_records: bool = True, graph: bool = True):
         plan = np.ascontiguousarray(plan, dtype=np.float64)
         n = int(plan.shape[0]) if plan.size else 0
         self._keep = [plan]
@@ -37,7 +37,8 @@ class DeviceRun:
         wr, wr_p = _u8(wrapped, n)
         du, du_p = _u8(dump, n)
         self._keep += [wr, du]
-        opt = RunOptions(precision, bc, T_bc, batch, depth, rows, T_ref, int(keep_records))
+        opt = RunOptions(precision, bc, T_bc, batch, depth, rows, T_ref, int(keep_records),
+                         int(graph), 0)
         h = C.c_void_p()
         rc = lib().lpbf_run_create(k, phys, Tm, Lx, Ly, Tamb, plan.ctypes.data if n else None,
                                    xc.ctypes.data if xc is not None else None, wr_p, du_p, n,
@@ -91,6 +92,9 @@ class DeviceRun:
         done = C.c_int()
         self._ok(lib().lpbf_run_summary(self._h, out, C.byref(done)), "lpbf_run_summary")
         return out[0], out[1], out[2], done.value
+
+    def graph_launches(self) -> int:
+        return lib().lpbf_run_graph_launches(self._h)
 
     def records(self, first: int = 0, n: int | None = None):
         if n is None:

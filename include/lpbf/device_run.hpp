@@ -13,7 +13,11 @@
 // finalised on the host in step order, exactly as MetricsTask does.
 //
 // Precision 64 runs the FP64 parity kernels, one step per launch, so every
-// per-step value equals the reference's.  Precision 32 runs the temporally
+// per-step value equals the reference's.  A full batch that starts at a
+// multiple of `batch` (even) and holds no snapshot is launched as one
+// captured hipGraph: the laser, ring slot and cruise gate of each step come
+// from device memory indexed by a device step counter, so the same graph
+// replays for every such batch (kernel nodes only).  Precision 32 runs the temporally
 // blocked FP32 kernel with up to `depth` steps per launch; a launch never
 // crosses a dump step or a batch end, and the hot column (hence W_sub) is
 // taken at the last step of each launch only.
@@ -38,6 +42,7 @@ struct RunOptions {
     int rows = 0;                  // FP32 band height (0 = built-in)
     double T_ref = -1.0;           // FP32 theta origin (< 0: Tamb)
     bool keep_records = true;      // keep one StepRecord per step
+    bool graph = true;             // FP64: replay full batches as one captured hipGraph
 };
 
 struct StepRecord {
@@ -80,9 +85,12 @@ public:
     std::vector<StepRecord> const& records() const noexcept { return rec_; }
     StepRecord const& last() const noexcept { return last_; }
     void* stream() const noexcept { return stream_; }
+    int graph_launches() const noexcept { return graph_launches_; }
 
 private:
     void batch(int bn);
+    bool graph_batch(int s0, int bn);
+    void capture(int parity);
     void release() noexcept;
     bool cruise(Tick const& t) const noexcept {
         return !t.wrapped && t.x_laser_csv >= 0.45 * dom_.Lx && t.x_laser_csv <= 0.85 * dom_.Lx;
@@ -110,6 +118,15 @@ private:
     double* d_snap_ = nullptr;     // FP32 path: FP64 staging for snapshots
     StepStats* h_ring_ = nullptr;  // pinned
     double* h_cols_ = nullptr;     // pinned
+    // FP64 graph replay: step counter and cruise flags on the device, one
+    // captured batch per physical buffer holding the current field
+    int* d_step_ = nullptr;
+    int* h_step_ = nullptr;        // pinned
+    unsigned char* d_cruise_ = nullptr;
+    void* buf0_ = nullptr;
+    void* graph_[2] = {nullptr, nullptr};
+    void* gexec_[2] = {nullptr, nullptr};
+    int graph_launches_ = 0;
     std::vector<StepRecord> rec_;
     StepRecord last_;
     int done_ = 0;

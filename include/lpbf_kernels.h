@@ -89,6 +89,23 @@ void lpbf_launch_step_ex(const double* Tin, double* Tout, const StepConst* k,
                          StepStats* block_stats, hipStream_t stream);
 int  lpbf_step_ex_nblocks(int nx, int ny);
 
+/* Plan-driven step for launch sequences that are captured once and
+ * replayed (hipGraph): the step index lives in device memory.  One launch
+ * does lpbf_launch_step_ex for step s = *d_step with the laser taken from
+ * d_plan[s] (k's x_l, y_l, q_scale ignored), writes its block records to
+ * block_stats (lpbf_step_ex_plan_nblocks(nx, ny) of them), and the last
+ * workgroup to finish merges them into ring[s % ring_len], copies column
+ * ring[..].arg % nx of Tout into cols[(s % ring_len) * ny ...] when cols and
+ * d_cruise are given and d_cruise[s] != 0, resets *d_ticket to 0 and sets
+ * *d_step = s + 1.  *d_ticket must be 0 before the first launch. */
+int  lpbf_step_ex_plan_nblocks(int nx, int ny);
+void lpbf_launch_step_ex_plan(const double* Tin, double* Tout, const StepConst* k,
+                              int phys, int bc, double T_bc, double Tm,
+                              const LaserTick* d_plan, int* d_step,
+                              const unsigned char* d_cruise, StepStats* block_stats,
+                              int* d_ticket, StepStats* ring, int ring_len, double* cols,
+                              hipStream_t stream);
+
 /* Merge block StepStats in a fixed order (deterministic FP64 sums). */
 void lpbf_launch_stats_finalize(const StepStats* parts, int nblocks, int nx,
                                 int ny, StepStats* out, hipStream_t stream);

@@ -28,6 +28,8 @@ typedef struct lpbf_run_options {
     int    rows;          /* FP32: band height, 0 = built-in */
     double T_ref;         /* FP32: theta origin [K], < 0 = Tamb */
     int    keep_records;  /* 1: keep one lpbf_step_record per step */
+    int    graph;         /* FP64: 1 = replay full batches as a captured hipGraph */
+    int    pad_;
 } lpbf_run_options;
 
 /* Per-step result: the merged statistics of the field after the step, the
@@ -63,6 +65,9 @@ int  lpbf_run_download(lpbf_run* run, double* T);                /* nx*ny FP64 *
 
 /* out[0..2] = peak T [K], peak W [um], steady W_sub [um]; *steps_done. */
 int  lpbf_run_summary(const lpbf_run* run, double* out3, int* steps_done);
+
+/* Number of batches that ran as a replayed hipGraph so far. */
+int  lpbf_run_graph_launches(const lpbf_run* run);
 
 /* Copies records [first, first+n) (requires keep_records). */
 int  lpbf_run_records(const lpbf_run* run, int first, int n, lpbf_step_record* out);

@@ -35,6 +35,19 @@ def _run(k, mat, pr, dom, plan, x_csv, wrapped, **kw):
                        wrapped=wrapped, **kw)
 
 
+def _same_records(ra, rb):
+    """Equal step records; a batch replayed as a graph uses shorter bands (more
+    workgroups), so only the FP64 energy sums may differ, in their last bits."""
+    assert len(ra) == len(rb)
+    for x, y in zip(ra, rb):
+        assert (x.step, x.has_col, x.W_um, x.W_sub_um) == (y.step, y.has_col, y.W_um, y.W_sub_um)
+        assert (x.stats.tmax, x.stats.arg, x.stats.ymin, x.stats.ymax, x.stats.molten) == \
+            (y.stats.tmax, y.stats.arg, y.stats.ymin, y.stats.ymax, y.stats.molten)
+        for f in ("sum_dT", "sum_src", "sum_loss"):
+            assert abs(getattr(x.stats, f) - getattr(y.stats, f)) <= \
+                1e-12 * (abs(getattr(x.stats, f)) + 1.0) * 240 * 120
+
+
 def test_case0_device_run_fp64_matches_oracle(ora):
     print("\n[case] DeviceRun case0 FP64 batch 64", flush=True)
     mat, pr, dom, dt, n, k, plan, x_csv, wrapped = _case0()
@@ -80,7 +93,29 @@ def test_batch_size_does_not_change_results(batch):
             got += b.advance(min(37, n - got))
         rb, Tb = b.records(), b.download()
     assert np.array_equal(Ta, Tb)
-    assert [bytes(r) for r in ra] == [bytes(r) for r in rb]
+    _same_records(ra, rb)
+
+
+@pytest.mark.parametrize("phys", [0, 1])
+def test_graph_batches_equal_eager_batches(phys):
+    print(f"\n[case] DeviceRun case0 FP64 phys={phys}: hipGraph batches vs eager", flush=True)
+    mat, pr, dom, dt, n, k, plan, x_csv, wrapped = _case0()
+    if phys:
+        dt = h.dt_cfl_ext(mat, dom, 1.0)
+        k = h.make_const(mat, pr, dom, dt)
+        plan, x_csv, wrapped = h.single_track_plan(pr.V, dt, dom.Lx, dom.Ly, n)
+    out = []
+    for graph in (False, True):
+        with _run(k, mat, pr, dom, plan, x_csv, wrapped, batch=64, phys=phys,
+                  graph=graph) as r:
+            r.advance(100)           # 64 aligned + 36: one graph batch, one eager
+            r.advance(n - 100)       # unaligned start: eager until the next multiple of 64
+            out.append((r.records(), r.download(), r.summary(), r.graph_launches()))
+    (ra, Ta, sa, ga), (rb, Tb, sb, gb) = out
+    assert ga == 0 and gb >= 15
+    assert np.array_equal(Ta, Tb)
+    _same_records(ra, rb)
+    assert sa == sb
 
 
 def test_dirichlet_noise_run_matches_oracle(ora):
