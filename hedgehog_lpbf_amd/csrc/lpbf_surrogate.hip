@@ -66,12 +66,24 @@ __device__ inline float bf2f(uint16_t h) {
     return __builtin_bit_cast(float, static_cast<uint32_t>(h) << 16);
 }
 
+// Hardware round-to-nearest-even conversion (v_cvt_pk_bf16_f32).
+__device__ inline uint32_t pack_bf2(float a, float b) {
+    return uint32_t(__builtin_bit_cast(uint16_t, static_cast<__bf16>(a))) |
+           (uint32_t(__builtin_bit_cast(uint16_t, static_cast<__bf16>(b))) << 16);
+}
+
+// tanh(u) = 1 - 2 / (1 + e^{2u}) with the hardware exp and reciprocal: a few
+// ulp of FP32, far below the bf16 rounding that follows (saturates to +-1).
+__device__ inline float fast_tanh(float u) {
+    return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __expf(2.f * u));
+}
+
 constexpr float kGeluC = 0.7978845608028654f;   // sqrt(2/pi)
 __device__ inline float gelu(float z) {
-    return 0.5f * z * (1.f + tanhf(kGeluC * (z + 0.044715f * z * z * z)));
+    return 0.5f * z * (1.f + fast_tanh(kGeluC * (z + 0.044715f * z * z * z)));
 }
 __device__ inline float gelu_grad(float z) {
-    const float t = tanhf(kGeluC * (z + 0.044715f * z * z * z));
+    const float t = fast_tanh(kGeluC * (z + 0.044715f * z * z * z));
     return 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * kGeluC * (1.f + 3.f * 0.044715f * z * z);
 }
 
@@ -114,6 +126,10 @@ __global__ void k_pack(const float* __restrict__ p, uint8_t* __restrict__ pack) 
 enum { kModeFwd = 0, kModeBwd = 1 };
 
 // X, out0, out1, zprev: [batch][128][128][32] bf16.  W: [9][32 out][32 in].
+// A workgroup owns two output rows of one window: it stages the four input
+// rows they touch (130 pixels with the zero border, 64 bytes each) in LDS
+// once, and each wavefront (one row half) reads its 16-byte operands from
+// there for all nine taps.
 template <int MODE>
 __global__ __launch_bounds__(kT) void k_conv_mfma(const uint16_t* __restrict__ X,
                                                   const uint16_t* __restrict__ W,
@@ -121,11 +137,33 @@ __global__ __launch_bounds__(kT) void k_conv_mfma(const uint16_t* __restrict__ X
                                                   const uint16_t* __restrict__ zprev,
                                                   uint16_t* __restrict__ out0,
                                                   uint16_t* __restrict__ out1, int batch) {
+    constexpr int kRowsIn = 4, kCols = kW + 2;
+    __shared__ __attribute__((aligned(16))) uint4 s_x[kRowsIn * kCols * 4];
     const int lane = threadIdx.x & 63;
-    const int gw = blockIdx.x * (kT / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (gw >= batch * kW * 2) return;
-    const int img = gw / (2 * kW), row = (gw >> 1) % kW, x0 = (gw & 1) * 64;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int img = blockIdx.x / (kW / 2), r0 = (blockIdx.x % (kW / 2)) * 2;
+    if (img >= batch) return;
     const int lr = lane & 15, lg = lane >> 4;
+
+    // all loads in flight before the first LDS store
+    const uint4* Xv = reinterpret_cast<const uint4*>(X) + static_cast<size_t>(img) * kPix * 4;
+    constexpr int kStage = kRowsIn * kCols * 4, kIt = (kStage + kT - 1) / kT;
+    uint4 v[kIt];
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+        const int q = threadIdx.x + it * kT;
+        const int rr = q / (kCols * 4), rem = q % (kCols * 4);
+        const int px = rem / 4 - 1, c = rem & 3;
+        const int yy = r0 - 1 + rr;
+        v[it] = make_uint4(0, 0, 0, 0);
+        if (q < kStage && yy >= 0 && yy < kW && px >= 0 && px < kW)
+            v[it] = Xv[(yy * kW + px) * 4 + c];
+    }
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+        const int q = threadIdx.x + it * kT;
+        if (q < kStage) s_x[q] = v[it];
+    }
 
     bf16x8 w[9][2];
 #pragma unroll
@@ -140,22 +178,16 @@ __global__ __launch_bounds__(kT) void k_conv_mfma(const uint16_t* __restrict__ X
     for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
         for (int pt = 0; pt < 4; ++pt) acc[ct][pt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    __syncthreads();
 
-    const uint16_t* Ximg = X + static_cast<size_t>(img) * kPix * kC;
+    const int lrow = wave >> 1, row = r0 + lrow, x0 = (wave & 1) * 64;
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
-        const int yy = row + t / 3 - 1;
-        if (yy < 0 || yy >= kW) continue;            // uniform: the whole row is padding
+        const int dy = t / 3, dx = t % 3;   // LDS row lrow + dy, column px + dx
         bf16x8 xv[4];
 #pragma unroll
-        for (int pt = 0; pt < 4; ++pt) {
-            const int xx = x0 + pt * 16 + lr + t % 3 - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (xx >= 0 && xx < kW)
-                v = *reinterpret_cast<const uint4*>(Ximg + (static_cast<size_t>(yy) * kW + xx) * kC +
-                                                    8 * lg);
-            xv[pt] = as_bf8(v);
-        }
+        for (int pt = 0; pt < 4; ++pt)
+            xv[pt] = as_bf8(s_x[((lrow + dy) * kCols + x0 + pt * 16 + lr + dx) * 4 + lg]);
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
@@ -171,27 +203,27 @@ __global__ __launch_bounds__(kT) void k_conv_mfma(const uint16_t* __restrict__ X
 #pragma unroll
         for (int pt = 0; pt < 4; ++pt) {
             const size_t off = (static_cast<size_t>(img) * kPix + row * kW + x0 + pt * 16 + lr) * kC + co0;
-            uint16_t o0[4], o1[4];
             if (MODE == kModeFwd) {
+                float z[4], a[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const float z = acc[ct][pt][i] + bias[co0 + i];
-                    o0[i] = f2bf(z);
-                    o1[i] = f2bf(gelu(z));
+                    z[i] = acc[ct][pt][i] + bias[co0 + i];
+                    a[i] = gelu(z[i]);
                 }
                 if (out0)
                     *reinterpret_cast<uint2*>(out0 + off) =
-                        make_uint2(o0[0] | (uint32_t(o0[1]) << 16), o0[2] | (uint32_t(o0[3]) << 16));
+                        make_uint2(pack_bf2(z[0], z[1]), pack_bf2(z[2], z[3]));
                 *reinterpret_cast<uint2*>(out1 + off) =
-                    make_uint2(o1[0] | (uint32_t(o1[1]) << 16), o1[2] | (uint32_t(o1[3]) << 16));
+                    make_uint2(pack_bf2(a[0], a[1]), pack_bf2(a[2], a[3]));
             } else {
                 const uint2 zp = *reinterpret_cast<const uint2*>(zprev + off);
                 const uint16_t zs[4] = {uint16_t(zp.x & 0xffff), uint16_t(zp.x >> 16),
                                         uint16_t(zp.y & 0xffff), uint16_t(zp.y >> 16)};
+                float d[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) o0[i] = f2bf(acc[ct][pt][i] * gelu_grad(bf2f(zs[i])));
+                for (int i = 0; i < 4; ++i) d[i] = acc[ct][pt][i] * gelu_grad(bf2f(zs[i]));
                 *reinterpret_cast<uint2*>(out0 + off) =
-                    make_uint2(o0[0] | (uint32_t(o0[1]) << 16), o0[2] | (uint32_t(o0[3]) << 16));
+                    make_uint2(pack_bf2(d[0], d[1]), pack_bf2(d[2], d[3]));
             }
         }
     }
@@ -224,26 +256,48 @@ __global__ __launch_bounds__(kT) void k_dw_mfma(const uint16_t* __restrict__ dz,
 
     for (int gr = r0; gr < r1; ++gr) {
         const int img = gr / kW, y = gr % kW;
-        __syncthreads();
-        for (int q = tid; q < kW * 4; q += kT) {
-            const int px = q >> 2, c = q & 3;
-            const uint4 v = *reinterpret_cast<const uint4*>(
-                dz + (static_cast<size_t>(img) * kPix + y * kW + px) * kC + c * 8);
-            uint2* d = reinterpret_cast<uint2*>(s_dz + px * kPitch + c * 8);
-            d[0] = make_uint2(v.x, v.y);
-            d[1] = make_uint2(v.z, v.w);
+        // all loads of the row in flight before the LDS stores
+        constexpr int kDz = kW * 4, kA = 3 * (kW + 2) * 4;
+        constexpr int kItD = (kDz + kT - 1) / kT, kItA = (kA + kT - 1) / kT;
+        uint4 vd[kItD], va[kItA];
+#pragma unroll
+        for (int it = 0; it < kItD; ++it) {
+            const int q = tid + it * kT, px = q >> 2, c = q & 3;
+            vd[it] = q < kDz ? *reinterpret_cast<const uint4*>(
+                                   dz + (static_cast<size_t>(img) * kPix + y * kW + px) * kC + c * 8)
+                             : make_uint4(0, 0, 0, 0);
         }
-        for (int q = tid; q < 3 * (kW + 2) * 4; q += kT) {
+#pragma unroll
+        for (int it = 0; it < kItA; ++it) {
+            const int q = tid + it * kT;
             const int rr = q / ((kW + 2) * 4), rem = q % ((kW + 2) * 4);
             const int px = rem / 4 - 1, c = rem & 3;
             const int yy = y + rr - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (yy >= 0 && yy < kW && px >= 0 && px < kW)
-                v = *reinterpret_cast<const uint4*>(
+            va[it] = make_uint4(0, 0, 0, 0);
+            if (q < kA && yy >= 0 && yy < kW && px >= 0 && px < kW)
+                va[it] = *reinterpret_cast<const uint4*>(
                     A + (static_cast<size_t>(img) * kPix + yy * kW + px) * kC + c * 8);
-            uint2* d = reinterpret_cast<uint2*>(s_a + (rr * (kW + 2) + px + 1) * kPitch + c * 8);
-            d[0] = make_uint2(v.x, v.y);
-            d[1] = make_uint2(v.z, v.w);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < kItD; ++it) {
+            const int q = tid + it * kT, px = q >> 2, c = q & 3;
+            if (q < kDz) {
+                uint2* d = reinterpret_cast<uint2*>(s_dz + px * kPitch + c * 8);
+                d[0] = make_uint2(vd[it].x, vd[it].y);
+                d[1] = make_uint2(vd[it].z, vd[it].w);
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < kItA; ++it) {
+            const int q = tid + it * kT;
+            if (q < kA) {
+                const int rr = q / ((kW + 2) * 4), rem = q % ((kW + 2) * 4);
+                const int px = rem / 4 - 1, c = rem & 3;
+                uint2* d = reinterpret_cast<uint2*>(s_a + (rr * (kW + 2) + px + 1) * kPitch + c * 8);
+                d[0] = make_uint2(va[it].x, va[it].y);
+                d[1] = make_uint2(va[it].z, va[it].w);
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -649,8 +703,7 @@ bool batch_ok(int b) { return b >= 1 && b <= 4096; }
 
 void conv_fwd(const uint16_t* X, const uint8_t* pack, int l, uint16_t* z, uint16_t* a, int batch,
               hipStream_t s) {
-    const int waves = batch * kW * 2;
-    hipLaunchKernelGGL(k_conv_mfma<kModeFwd>, dim3((waves + 3) / 4), dim3(kT), 0, s, X,
+    hipLaunchKernelGGL(k_conv_mfma<kModeFwd>, dim3(batch * (kW / 2)), dim3(kT), 0, s, X,
                        reinterpret_cast<const uint16_t*>(pack + kOffWF) + static_cast<size_t>(l) * kWsz,
                        reinterpret_cast<const float*>(pack + kOffB) + l * kC,
                        static_cast<const uint16_t*>(nullptr), z, a, batch);
@@ -727,7 +780,6 @@ void lpbf_sg_loss_grad(const void* d_pack, const void* d_x0, const float* d_targ
                        w.dz[0], batch);
     // layers 5..1
     int cur = 0;
-    const int waves = batch * kW * 2;
     for (int l = 4; l >= 0; --l) {
         const uint16_t* a_prev = l > 0 ? w.act[5 + l - 1] : x0;
         hipLaunchKernelGGL(k_dw_mfma, dim3(nwg), dim3(kT), 0, stream, w.dz[cur], a_prev, w.part,
@@ -735,7 +787,7 @@ void lpbf_sg_loss_grad(const void* d_pack, const void* d_x0, const float* d_targ
         hipLaunchKernelGGL(k_dw_reduce, dim3((kDwSlots * kC * kC + kT - 1) / kT), dim3(kT), 0,
                            stream, w.part, nwg, l, d_grad);
         if (l > 0) {
-            hipLaunchKernelGGL(k_conv_mfma<kModeBwd>, dim3((waves + 3) / 4), dim3(kT), 0, stream,
+            hipLaunchKernelGGL(k_conv_mfma<kModeBwd>, dim3(batch * (kW / 2)), dim3(kT), 0, stream,
                                w.dz[cur],
                                reinterpret_cast<const uint16_t*>(pack + kOffWB) +
                                    static_cast<size_t>(l) * kWsz,
