@@ -334,33 +334,31 @@ __global__ __launch_bounds__(kT) void k_dw_mfma(const uint16_t* __restrict__ dz,
 }
 
 // Sum the per-workgroup partials (fixed order) and scatter into the FP32
-// gradient vector of layer l (0-based, 0..4).
-__global__ void k_dw_reduce(const float* __restrict__ partial, int nwg, int l,
-                            float* __restrict__ grad) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;   // slot*1024 + co*32 + ci
-    if (j >= kDwSlots * kC * kC) return;
-    const int s = j / (kC * kC), co = (j / kC) % kC, ci = j % kC;
-    if (s == 9 && ci != 0) return;
-    if (l == 0 && s < 9 && ci >= 4) return;
-    // four independent chains (fixed order: deterministic) hide the load latency
+// gradient vector of layer l (0-based, 0..4).  A block takes 32 entries;
+// thread (o, g) sums workgroups g, g+8, ..., then a fixed tree over g.
+__global__ __launch_bounds__(kT) void k_dw_reduce(const float* __restrict__ partial, int nwg,
+                                                  int l, float* __restrict__ grad) {
+    __shared__ float s[8][32];
+    const int o = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const int j = blockIdx.x * 32 + o;                     // slot*1024 + co*32 + ci
     const size_t stride = static_cast<size_t>(kDwSlots) * kC * kC;
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-    int w = 0;
-    for (; w + 4 <= nwg; w += 4) {
-        s0 += partial[(w + 0) * stride + j];
-        s1 += partial[(w + 1) * stride + j];
-        s2 += partial[(w + 2) * stride + j];
-        s3 += partial[(w + 3) * stride + j];
-    }
-    for (; w < nwg; ++w) s0 += partial[w * stride + j];
-    const float sum = (s0 + s1) + (s2 + s3);
+    float a = 0.f;
+    for (int w = g; w < nwg; w += 8) a += partial[w * stride + j];
+    s[g][o] = a;
+    __syncthreads();
+    if (g != 0) return;
+    const float sum = ((s[0][o] + s[1][o]) + (s[2][o] + s[3][o])) +
+                      ((s[4][o] + s[5][o]) + (s[6][o] + s[7][o]));
+    const int sl = j / (kC * kC), co = (j / kC) % kC, ci = j % kC;
+    if (sl == 9 && ci != 0) return;
+    if (l == 0 && sl < 9 && ci >= 4) return;
     if (l == 0) {
-        if (s < 9) grad[(s * 4 + ci) * kC + co] = sum;
+        if (sl < 9) grad[(sl * 4 + ci) * kC + co] = sum;
         else grad[kL1W + co] = sum;
     } else {
-        float* g = grad + kL1 + (l - 1) * kLn;
-        if (s < 9) g[(s * kC + ci) * kC + co] = sum;
-        else g[kLW + co] = sum;
+        float* gp = grad + kL1 + (l - 1) * kLn;
+        if (sl < 9) gp[(sl * kC + ci) * kC + co] = sum;
+        else gp[kLW + co] = sum;
     }
 }
 
@@ -489,19 +487,33 @@ __global__ __launch_bounds__(kT) void k_out_dw(const uint16_t* __restrict__ a5,
     float accb = 0.f;
     for (int gr = r0; gr < r1; ++gr) {
         const int img = gr / kW, y = gr % kW;
-        __syncthreads();
-        if (tid < kW) s_dy[tid] = dy[static_cast<size_t>(img) * kPix + y * kW + tid];
-        for (int q = tid; q < 3 * (kW + 2) * 4; q += kT) {
+        // all loads of the row in flight before the LDS stores
+        constexpr int kA = 3 * (kW + 2) * 4, kItA = (kA + kT - 1) / kT;
+        uint4 va[kItA];
+        const float dyv = tid < kW ? dy[static_cast<size_t>(img) * kPix + y * kW + tid] : 0.f;
+#pragma unroll
+        for (int it = 0; it < kItA; ++it) {
+            const int q = tid + it * kT;
             const int rr = q / ((kW + 2) * 4), rem = q % ((kW + 2) * 4);
             const int px = rem / 4 - 1, c = rem & 3;
             const int yy = y + rr - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (yy >= 0 && yy < kW && px >= 0 && px < kW)
-                v = *reinterpret_cast<const uint4*>(
+            va[it] = make_uint4(0, 0, 0, 0);
+            if (q < kA && yy >= 0 && yy < kW && px >= 0 && px < kW)
+                va[it] = *reinterpret_cast<const uint4*>(
                     a5 + (static_cast<size_t>(img) * kPix + yy * kW + px) * kC + c * 8);
-            uint2* d = reinterpret_cast<uint2*>(s_a + (rr * (kW + 2) + px + 1) * kPitch + c * 8);
-            d[0] = make_uint2(v.x, v.y);
-            d[1] = make_uint2(v.z, v.w);
+        }
+        __syncthreads();
+        if (tid < kW) s_dy[tid] = dyv;
+#pragma unroll
+        for (int it = 0; it < kItA; ++it) {
+            const int q = tid + it * kT;
+            if (q < kA) {
+                const int rr = q / ((kW + 2) * 4), rem = q % ((kW + 2) * 4);
+                const int px = rem / 4 - 1, c = rem & 3;
+                uint2* d = reinterpret_cast<uint2*>(s_a + (rr * (kW + 2) + px + 1) * kPitch + c * 8);
+                d[0] = make_uint2(va[it].x, va[it].y);
+                d[1] = make_uint2(va[it].z, va[it].w);
+            }
         }
         __syncthreads();
         if (tid < 3 * kC) {
@@ -784,8 +796,8 @@ void lpbf_sg_loss_grad(const void* d_pack, const void* d_x0, const float* d_targ
         const uint16_t* a_prev = l > 0 ? w.act[5 + l - 1] : x0;
         hipLaunchKernelGGL(k_dw_mfma, dim3(nwg), dim3(kT), 0, stream, w.dz[cur], a_prev, w.part,
                            rows, per);
-        hipLaunchKernelGGL(k_dw_reduce, dim3((kDwSlots * kC * kC + kT - 1) / kT), dim3(kT), 0,
-                           stream, w.part, nwg, l, d_grad);
+        hipLaunchKernelGGL(k_dw_reduce, dim3(kDwSlots * kC * kC / 32), dim3(kT), 0, stream,
+                           w.part, nwg, l, d_grad);
         if (l > 0) {
             hipLaunchKernelGGL(k_conv_mfma<kModeBwd>, dim3(batch * (kW / 2)), dim3(kT), 0, stream,
                                w.dz[cur],
