@@ -224,6 +224,34 @@ void ora_step(const double* Tin, double* Tout, const StepConst* k, int phys) {
     ora_step_ex(Tin, Tout, k, phys, LPBF_BC_MIRROR, 0.0, 0.0, NULL);
 }
 
+/* ora_step restricted to rows [y0, y1): the same cell arithmetic, so row
+ * ranges stepped by separate threads give the ora_step field exactly. */
+void ora_step_rows(const double* Tin, double* Tout, const StepConst* k, int phys,
+                   int y0, int y1) {
+    const int nx = k->nx, ny = k->ny;
+    if (y0 < 0) y0 = 0;
+    if (y1 > ny) y1 = ny;
+    for (int y = y0; y < y1; ++y) {
+        const double ypos = (double)y * k->dy;
+        for (int x = 0; x < nx; ++x) {
+            const long long c = (long long)y * nx + x;
+            const double T = Tin[c];
+            const double xpos = (double)x * k->dx;
+            double src = 0.0, lossv = 0.0;
+            const double Txm = (x == 0) ? T : Tin[c - 1];
+            const double Txp = (x == nx - 1) ? T : Tin[c + 1];
+            const double Tym = (y == 0) ? T : Tin[c - nx];
+            const double Typ = (y == ny - 1) ? T : Tin[c + nx];
+            if (phys == LPBF_PHYS_BASELINE) {
+                const int xe = (x == 0) ? -1 : (x == nx - 1 ? 1 : 0);
+                Tout[c] = cell_base(T, Txm, Txp, Tym, Typ, xe, xpos, ypos, k, &src, &lossv);
+            } else {
+                Tout[c] = cell_ext(T, Txm, Txp, Tym, Typ, xpos, ypos, k, &src, &lossv);
+            }
+        }
+    }
+}
+
 void ora_step_ex(const double* Tin, double* Tout, const StepConst* k,
                  int phys, int bc, double T_bc, double Tm, StepStats* st) {
     const int nx = k->nx, ny = k->ny;

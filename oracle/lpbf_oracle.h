@@ -82,6 +82,10 @@ void   ora_make_const(const OraMaterial* m, const OraProcess* p,
 
 /* Full-grid step with ghost = centre on every edge (lpbf_cuda.cu:154-210). */
 void   ora_step(const double* Tin, double* Tout, const StepConst* k, int phys);
+/* ora_step on rows [y0, y1) only (mirror edges); used to time the oracle on
+ * several host threads. */
+void   ora_step_rows(const double* Tin, double* Tout, const StepConst* k, int phys,
+                     int y0, int y1);
 
 /* Row-strip tile step (step_task.hpp:54-206).  `tile` holds rows+2 rows of
  * nx values: halo row 0, owned rows 1..rows, halo row rows+1.  Owned rows

@@ -110,6 +110,7 @@ def load() -> C.CDLL:
         "ora_cp_eff": (d, [d]), "ora_alpha_eff_max": (d, [d, d]),
         "ora_make_const": (v, [C.c_void_p, C.c_void_p, C.c_void_p, d, C.c_void_p]),
         "ora_step": (v, [_DP, _DP, C.c_void_p, i]),
+        "ora_step_rows": (v, [_DP, _DP, C.c_void_p, i, i, i]),
         "ora_step_ex": (v, [_DP, _DP, C.c_void_p, i, i, d, d, C.c_void_p]),
         "ora_step_tile": (v, [_DP, _DP, i, i, C.c_void_p, i]),
         "ora_step_tiled": (v, [_DP, _DP, C.c_void_p, i, i]),
@@ -179,6 +180,26 @@ def make_const(s: Setup, dt: float) -> StepConst:
 def step(T: np.ndarray, k: StepConst, phys: int = 0) -> np.ndarray:
     out = np.empty_like(T)
     lib().ora_step(_dp(T), _dp(out), C.byref(k), phys)
+    return out
+
+
+def step_threads(T: np.ndarray, k: StepConst, phys: int = 0, threads: int = 1,
+                 out: np.ndarray | None = None, pool=None) -> np.ndarray:
+    """ora_step with the rows split over `threads` host threads (the ctypes
+    calls release the GIL); equals step() exactly."""
+    out = np.empty_like(T) if out is None else out
+    ny = T.shape[0]
+    cuts = [ny * t // threads for t in range(threads + 1)]
+    pT, pO, pk = _dp(T), _dp(out), C.byref(k)
+
+    def part(t):
+        lib().ora_step_rows(pT, pO, pk, phys, cuts[t], cuts[t + 1])
+
+    if pool is None or threads == 1:
+        for t in range(threads):
+            part(t)
+    else:
+        list(pool.map(part, range(threads)))
     return out
 
 

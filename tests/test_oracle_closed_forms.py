@@ -363,3 +363,16 @@ def test_raster_positions(ora):
     # after the last track the beam stays off at the end point
     assert np.all(p[140:, 2] == 0.0)
     np.testing.assert_allclose(p[199, :2], [1.4e-3, 2e-3 + 1.6e-4], rtol=1e-12)
+
+
+def test_threaded_rows_equal_single_step(ora):
+    """The threaded timing path of the oracle computes exactly ora_step."""
+    from concurrent.futures import ThreadPoolExecutor
+    s = ora.defaults("0")
+    k = ora.make_const(s, ora.dt_cfl(s))
+    k.x_l, k.y_l = 0.5e-3, 0.3e-3
+    rng = np.random.default_rng(3)
+    T = 300.0 + 2000.0 * rng.random((s.dom.ny, s.dom.nx))
+    with ThreadPoolExecutor(5) as pool:
+        got = ora.step_threads(T, k, 0, threads=5, pool=pool)
+    assert np.array_equal(got, ora.step(T, k, 0))
