@@ -16,14 +16,19 @@
 // is the only one.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <filesystem>
 #include <fstream>
 #include <iostream>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lpbf/device_run.hpp"
@@ -52,6 +57,7 @@ struct CLI {
     bool do_bench = false, quiet = false;
     std::string manifest;
     int precision = 64, depth = 4, rows = 0;
+    int jobs = 1;                  // manifest rows run concurrently (one stream each)
     std::string bc = "mirror";
     double T_bc = 300.0;
 };
@@ -75,7 +81,9 @@ struct CLI {
            "  --device=N           HIP device (default 0)\n"
            "  --bench              print wall ms + steps/s (no file output)\n"
            "  --quiet              emit one compact RESULT csv line (no disk output)\n"
-           "  --manifest=FILE      loop a runs CSV in-process\n";
+           "  --manifest=FILE      loop a runs CSV in-process\n"
+           "  --jobs=N             run N manifest rows at once, each on its own stream\n"
+           "                       (RESULT lines still in manifest order)\n";
     std::exit(rc);
 }
 
@@ -114,6 +122,7 @@ CLI parse_cli(int argc, char** argv) {
         else if (key == "--rows") c.rows = n;
         else if (key == "--bc") c.bc = v;
         else if (key == "--Tbc") c.T_bc = f;
+        else if (key == "--jobs") c.jobs = n < 1 ? 1 : n;
         else if (a == "--help" || a == "-h") usage(0);
         else {
             std::cerr << "unknown option " << a << "\n";
@@ -255,12 +264,12 @@ int run_manifest(CLI const& cli) {
                 "dsub_m,kmult,steady_W_sub_um,peak_T_K,wall_ms,n_steps\n");
     std::fflush(stdout);
     std::string line;
-    long done = 0;
+    std::vector<RunSpec> rows;
     while (std::getline(f, line)) {
         if (line.empty()) continue;
         const std::vector<std::string> v = split_csv(line);
         if (static_cast<int>(v.size()) < need) {
-            std::cerr << "manifest row " << done + 1 << " has " << v.size() << " fields\n";
+            std::cerr << "manifest row " << rows.size() + 1 << " has " << v.size() << " fields\n";
             return 4;
         }
         RunSpec rs;
@@ -269,10 +278,59 @@ int run_manifest(CLI const& cli) {
         rs.P = std::atof(v[i_P].c_str()); rs.V = std::atof(v[i_V].c_str());
         rs.r0 = std::atof(v[i_r0].c_str()); rs.eta = std::atof(v[i_eta].c_str());
         rs.dsub_m = std::atof(v[i_dsub].c_str()); rs.kmult = std::atof(v[i_km].c_str());
-        emit_result_line(rs, run_one_row(rs, cli.t_end, cli));
-        ++done;
+        rows.push_back(rs);
     }
-    std::fprintf(stderr, "[manifest] completed %ld runs\n", done);
+    if (cli.jobs <= 1) {
+        for (RunSpec const& rs : rows) emit_result_line(rs, run_one_row(rs, cli.t_end, cli));
+    } else {
+        // worker threads take rows in order; each row owns a DeviceRun and its
+        // stream, so small plates share the GPU; lines are printed in order
+        std::vector<RowResult> res(rows.size());
+        std::vector<std::exception_ptr> err(rows.size());
+        std::vector<char> ready(rows.size(), 0);
+        std::mutex m;
+        std::condition_variable cv;
+        std::atomic<std::size_t> next{0};
+        std::vector<std::thread> pool;
+        const int nthreads = std::min<int>(cli.jobs, static_cast<int>(rows.size()));
+        for (int t = 0; t < nthreads; ++t)
+            pool.emplace_back([&] {
+                const bool dev_ok = hipSetDevice(cli.device) == hipSuccess;
+                for (std::size_t i; (i = next.fetch_add(1)) < rows.size();) {
+                    RowResult r{};
+                    std::exception_ptr e;
+                    try {
+                        if (!dev_ok) throw std::runtime_error("hipSetDevice failed in a worker");
+                        r = run_one_row(rows[i], cli.t_end, cli);
+                    } catch (...) {
+                        e = std::current_exception();
+                    }
+                    {
+                        std::lock_guard<std::mutex> g(m);
+                        res[i] = r;
+                        err[i] = e;
+                        ready[i] = 1;
+                    }
+                    cv.notify_all();
+                }
+            });
+        int rc = 0;
+        for (std::size_t i = 0; i < rows.size(); ++i) {
+            std::unique_lock<std::mutex> g(m);
+            cv.wait(g, [&] { return ready[i] != 0; });
+            if (err[i]) {
+                try { std::rethrow_exception(err[i]); }
+                catch (std::exception const& e) { std::cerr << "lpbf_gpu: row " << i + 1 << ": " << e.what() << "\n"; }
+                rc = 2;
+                continue;
+            }
+            g.unlock();
+            emit_result_line(rows[i], res[i]);
+        }
+        for (auto& t : pool) t.join();
+        if (rc) return rc;
+    }
+    std::fprintf(stderr, "[manifest] completed %zu runs\n", rows.size());
     return 0;
 }
 

@@ -220,6 +220,12 @@ def test_lpbf_gpu_manifest_and_quiet(tmp_path, ora):
         assert int(row[15]) == n
         assert row[13] == "%.6g" % summ[0]          # peak_T_K
         assert row[12] == "%.6g" % summ[2]          # steady_W_sub_um
+    # the same rows concurrently (one stream each): identical RESULT values, same order
+    rj = subprocess.run([EXE, "--manifest=%s" % man, "--tend=4e-4", "--jobs=2"],
+                        capture_output=True, text=True, timeout=300)
+    assert rj.returncode == 0, rj.stderr
+    rows_j = [ln.split(",") for ln in rj.stdout.splitlines() if ln.startswith("RESULT,")]
+    assert [x[:14] + x[15:] for x in rows_j] == [x[:14] + x[15:] for x in rows]
     q = subprocess.run([EXE, "--case=0", "--quiet", "--tend=4e-4"], capture_output=True,
                        text=True, timeout=300)
     assert q.returncode == 0, q.stderr
