@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -49,6 +50,12 @@ void lib_ok(const char* what) {
 
 hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
 
+// Runs on several host threads (one stream each) may share the device.  A
+// graph capture must not overlap another thread's allocation, free or
+// device-wide call, so capture, set-up and teardown take this lock; the
+// steady state (graph launches, batch copies and syncs) does not.
+std::mutex g_setup;
+
 }  // namespace
 
 DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<Tick> plan,
@@ -73,6 +80,7 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
 
     const std::size_t N = static_cast<std::size_t>(nx_) * ny_;
     const std::size_t B = static_cast<std::size_t>(opt_.batch);
+    std::lock_guard<std::mutex> lock(g_setup);
     try {
         hipStream_t s;
         hip_ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
@@ -89,16 +97,18 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
                 std::vector<LaserTick> lt = to_laser(plan_);
                 hip_ok(hipMalloc(reinterpret_cast<void**>(&d_plan_), lt.size() * sizeof(LaserTick)),
                        "plan");
-                hip_ok(hipMemcpy(d_plan_, lt.data(), lt.size() * sizeof(LaserTick),
-                                 hipMemcpyHostToDevice), "plan upload");
+                hip_ok(hipMemcpyAsync(d_plan_, lt.data(), lt.size() * sizeof(LaserTick),
+                                      hipMemcpyHostToDevice, S(stream_)), "plan upload");
+                hip_ok(hipStreamSynchronize(S(stream_)), "plan upload");
                 std::vector<unsigned char> cr(plan_.size());
                 for (std::size_t i = 0; i < plan_.size(); ++i) cr[i] = cruise(plan_[i]) ? 1 : 0;
                 hip_ok(hipMalloc(reinterpret_cast<void**>(&d_cruise_), cr.size()), "cruise flags");
-                hip_ok(hipMemcpy(d_cruise_, cr.data(), cr.size(), hipMemcpyHostToDevice),
-                       "cruise upload");
+                hip_ok(hipMemcpyAsync(d_cruise_, cr.data(), cr.size(), hipMemcpyHostToDevice,
+                                      S(stream_)), "cruise upload");
+                hip_ok(hipStreamSynchronize(S(stream_)), "cruise upload");
                 hip_ok(hipMalloc(reinterpret_cast<void**>(&d_step_), 2 * sizeof(int)),
                        "step counter and ticket");
-                hip_ok(hipMemset(d_step_, 0, 2 * sizeof(int)), "ticket");
+                hip_ok(hipMemsetAsync(d_step_, 0, 2 * sizeof(int), S(stream_)), "ticket");
                 const int pb = lpbf_step_ex_plan_nblocks(nx_, ny_);
                 if (pb > nblk_) {
                     hip_ok(hipFree(d_blk_), "block records");
@@ -121,8 +131,9 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
             if (!lt.empty()) {
                 hip_ok(hipMalloc(reinterpret_cast<void**>(&d_plan_), lt.size() * sizeof(LaserTick)),
                        "plan");
-                hip_ok(hipMemcpy(d_plan_, lt.data(), lt.size() * sizeof(LaserTick),
-                                 hipMemcpyHostToDevice), "plan upload");
+                hip_ok(hipMemcpyAsync(d_plan_, lt.data(), lt.size() * sizeof(LaserTick),
+                                      hipMemcpyHostToDevice, S(stream_)), "plan upload");
+                hip_ok(hipStreamSynchronize(S(stream_)), "plan upload");
             }
         }
         hip_ok(hipMalloc(reinterpret_cast<void**>(&d_ring_), B * sizeof(StepStats)), "ring");
@@ -135,16 +146,21 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
         std::vector<double> init(N, Tamb_);
         upload(init.data());
     } catch (...) {
-        release();
+        release_locked();
         throw;
     }
 }
 
 DeviceRun::~DeviceRun() { release(); }
 
+void DeviceRun::release() noexcept {
+    std::lock_guard<std::mutex> lock(g_setup);
+    release_locked();
+}
+
 // Teardown cannot report failures; a sticky device error stays visible to
 // the next HIP call of the caller.
-void DeviceRun::release() noexcept {
+void DeviceRun::release_locked() noexcept {
     if (stream_) (void)hipStreamSynchronize(S(stream_));
     for (int q = 0; q < 2; ++q) {
         if (gexec_[q]) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(gexec_[q]));
@@ -267,6 +283,7 @@ void DeviceRun::finish_step(Tick const& tk, StepStats const& st, const double* c
 void DeviceRun::capture(int parity) {
     const hipStream_t s = S(stream_);
     const int B = opt_.batch;
+    std::lock_guard<std::mutex> lock(g_setup);
     double* cur = static_cast<double*>(parity == 0 ? buf0_ : (buf0_ == d_a_ ? d_b_ : d_a_));
     double* nxt = static_cast<double*>(cur == d_a_ ? d_b_ : d_a_);
     hipGraph_t g = nullptr;

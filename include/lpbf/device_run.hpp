@@ -92,6 +92,7 @@ private:
     bool graph_batch(int s0, int bn);
     void capture(int parity);
     void release() noexcept;
+    void release_locked() noexcept;
     bool cruise(Tick const& t) const noexcept {
         return !t.wrapped && t.x_laser_csv >= 0.45 * dom_.Lx && t.x_laser_csv <= 0.85 * dom_.Lx;
     }
