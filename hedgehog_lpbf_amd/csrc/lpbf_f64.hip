@@ -157,6 +157,24 @@ struct Region {
     double* cols;
 };
 
+// Whole-wavefront lane moves of a double without LDS: DPP wave_shr:1 /
+// wave_shl:1 on both halves (lane l receives lane l-1 / l+1; the lane
+// without a source receives 0), and a lane broadcast through readlane.
+__device__ __forceinline__ double lane_from_left(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_from_right(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_read(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
 template <int PHYS, int RED>
 __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                                                        void* red_out, int red_fill) {
@@ -208,17 +226,29 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
             }
         }
 
+        // row y+1 is fetched one iteration ahead, so its load overlaps the
+        // arithmetic of row y (the next row's values are only moved, never
+        // recomputed: the cell arithmetic is unchanged)
+        double f0 = 0.0, f1 = 0.0, fe = 0.0;
+        if (ya < yb && (ya + 1 < ny || tile)) {
+            const double* rn = g.in + static_cast<long long>(ya + 1) * nx;
+            f0 = a0 ? rn[x0] : 0.0;
+            f1 = a1 ? rn[x1] : 0.0;
+            fe = eok ? rn[ecol] : 0.0;
+        }
         for (int y = ya; y < yb; ++y) {
-            double n0v = c0v, n1v = c1v, ne = ce;
-            if (y + 1 < ny || tile) {
-                const double* rn = g.in + static_cast<long long>(y + 1) * nx;
-                n0v = a0 ? rn[x0] : 0.0;
-                n1v = a1 ? rn[x1] : 0.0;
-                ne = eok ? rn[ecol] : 0.0;
+            const bool has_next = (y + 1 < ny || tile);
+            const double n0v = has_next ? f0 : c0v, n1v = has_next ? f1 : c1v;
+            const double ne = has_next ? fe : ce;
+            if (y + 1 < yb && (y + 2 < ny || tile)) {
+                const double* rn = g.in + static_cast<long long>(y + 2) * nx;
+                f0 = a0 ? rn[x0] : 0.0;
+                f1 = a1 ? rn[x1] : 0.0;
+                fe = eok ? rn[ecol] : 0.0;
             }
-            const double up0 = __shfl_up(c0v, 1), up1 = __shfl_up(c1v, 1);
-            const double dn0 = __shfl_down(c0v, 1), dn1 = __shfl_down(c1v, 1);
-            const double last0 = __shfl(c0v, kWave - 1), first1 = __shfl(c1v, 0);
+            const double up0 = lane_from_left(c0v), up1 = lane_from_left(c1v);
+            const double dn0 = lane_from_right(c0v), dn1 = lane_from_right(c1v);
+            const double last0 = lane_read(c0v, kWave - 1), first1 = lane_read(c1v, 0);
             const double xm[2] = {lane ? up0 : ce, lane ? up1 : last0};
             const double xp[2] = {lane < kWave - 1 ? dn0 : first1,
                                   lane < kWave - 1 ? dn1 : ce};

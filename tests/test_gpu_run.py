@@ -200,6 +200,21 @@ def test_lpbf_gpu_driver_writes_reference_formats(tmp_path, ora):
     assert np.abs(T - T_o).max() <= 1e-9
 
 
+def test_lpbf_gpu_fp32_metrics_within_half_kelvin(tmp_path):
+    print("\n[case] lpbf_gpu case0 --precision=32", flush=True)
+    out = tmp_path / "case0_f32"
+    r = subprocess.run([EXE, "--case=0", "--snap-every=200", "--precision=32", f"--out={out}"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    got = [ln.split(",") for ln in (out / "metrics.csv").read_text().splitlines()[1:]]
+    want = [ln.split(",") for ln in open(os.path.join(GOLD, "case0_metrics.csv")).read()
+            .splitlines()[1:]]
+    assert [g[0] for g in got] == [w[0] for w in want]          # the same dump steps
+    for g, w in zip(got, want):
+        assert g[1:3] == w[1:3]                                 # time and laser position
+        assert abs(float(g[3]) - float(w[3])) <= 0.5 + 0.01     # peak T (printed to 6 digits)
+
+
 def test_lpbf_gpu_manifest_and_quiet(tmp_path, ora):
     print("\n[case] lpbf_gpu --manifest (2 rows) and --quiet", flush=True)
     man = tmp_path / "runs.csv"
