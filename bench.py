@@ -82,11 +82,15 @@ def cpu_baseline(k, nx, ny, target_s=12.0):
                       f"{nx}x{ny} plate, {steps} steps, {el:.1f} s"}
 
 
-def traffic_for(workload):
+def traffic_for(workload, **launch):
+    """HBM bytes per launch from the committed counter summary, only when it
+    was measured on the same launch shape (e.g. depth and band rows)."""
     p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(p):
         with open(p) as f:
-            return json.load(f).get("bytes_per_launch")
+            t = json.load(f)
+        if all(t.get(k) == v for k, v in launch.items()):
+            return t.get("bytes_per_launch")
     return None
 
 
@@ -557,7 +561,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_for(args.workload),
+            "traffic": (traffic_for(args.workload, depth=depth, rows=rows)
+                        if args.workload == "plate8k_f32" else traffic_for(args.workload)),
             "alg_bytes_per_launch": alg_bytes,
             "kernel_ms_per_launch": kern_ms,
         },

@@ -123,6 +123,13 @@ CLI parse_cli(int argc, char** argv) {
         else if (key == "--bc") c.bc = v;
         else if (key == "--Tbc") c.T_bc = f;
         else if (key == "--jobs") c.jobs = n < 1 ? 1 : n;
+        else if (key == "--check" || key == "--check-rtol" || key == "--check-atol") {
+            // the reference binary runs its CPU stepper here (lpbf_cuda.cu:485-514);
+            // this binary carries no CPU solver
+            std::cerr << a << ": the CPU-vs-GPU report is `python -m tests.check_run` "
+                                "(same run options, --check-rtol/--check-atol)\n";
+            std::exit(1);
+        }
         else if (a == "--help" || a == "-h") usage(0);
         else {
             std::cerr << "unknown option " << a << "\n";
