@@ -8,7 +8,7 @@ namespace lpbf_amd {
 namespace {
 
 template <int K, int BC, bool RAD>
-__global__ __launch_bounds__(kThreads) void k_step_f32(F32Args a,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_step_f32(F32Args a,
                                                        const LaserTick* __restrict__ plan,
                                                        int step0) {
     step_tile<K, BC, RAD>(a, nullptr, nullptr, plan, step0, xcd_remap(blockIdx.x, a.nwg));
@@ -178,25 +178,27 @@ int lpbf_f32_autotune(const StepConst* k, double T_ref, int bc, double T_bc, dou
                 if (R > ny || (R > 0 && R < 2 * depth)) continue;
                 lpbf_launch_step_f32(a, b, ld, k, T_ref, bc, T_bc, Tm, d_plan, 0, depth, R, nullptr,
                                      work, stream);   // warm-up
-                hipEventRecord(e0, stream);
+                err = hipEventRecord(e0, stream);
+                if (err != hipSuccess) break;
                 for (int q = 0; q < reps; ++q)
                     lpbf_launch_step_f32((q & 1) ? a : b, (q & 1) ? b : a, ld, k, T_ref, bc, T_bc,
                                          Tm, d_plan, 0, depth, R, nullptr, work, stream);
-                hipEventRecord(e1, stream);
-                err = hipEventSynchronize(e1);
-                if (err != hipSuccess) break;
+                err = hipEventRecord(e1, stream);
+                if (err == hipSuccess) err = hipEventSynchronize(e1);
                 float ms = 0.f;
-                hipEventElapsedTime(&ms, e0, e1);
+                if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+                if (err != hipSuccess) break;
                 const double per_step = ms / (reps * depth);
                 if (per_step < best) { best = per_step; bd = depth; br = R; }
             }
         }
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    if (a) hipFree(a);
-    if (b) hipFree(b);
-    if (work) hipFree(work);
+    // release everything; the first error (if any) is the one reported
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (work) (void)hipFree(work);
     if (err != hipSuccess) {
         record_error(static_cast<int>(err));
         return static_cast<int>(err);

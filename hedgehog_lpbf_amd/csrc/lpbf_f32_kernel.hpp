@@ -53,6 +53,7 @@ struct F32Args {
     int nstrips, R, ntasks, nwg;
     int bc;
     float cx, cy;          // alpha*dt/dx^2, alpha*dt/dy^2
+    float kc;              // 1 - 2*cx - 2*cy, the centre weight of the update
     // dt*loss(theta) in K = (((a4*th + a3)*th + a2)*th + a1)*th + a0 (Horner):
     // convection + substrate are linear in theta, radiation expands
     // (T_ref + theta)^4 - Tamb^4 around T_ref.
@@ -65,6 +66,16 @@ struct F32Args {
     double Tref_d;
     StepStats* part;       // [depth][nwg]
 };
+
+// Largest of four finite values: v_max3 + v_max, without the NaN-quieting
+// v_max x,x the compiler inserts before fmaxf of values it cannot prove
+// canonical (no operand here can be a NaN).
+__device__ __forceinline__ float max4(float4 v) {
+    float m, r;
+    asm("v_max3_f32 %0, %1, %2, %3" : "=v"(m) : "v"(v.x), "v"(v.y), "v"(v.z));
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(m), "v"(v.w));
+    return r;
+}
 
 __device__ inline int xcd_remap(int bid, int n) {
     const int per = n / kXcds, rem = n % kXcds;
@@ -155,7 +166,11 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
     LevelAcc acc[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j] = LevelAcc{kNegInf, 0x7fffffff, ny, -1, 0, 0.0, 0.f, 0.f};
-    double S0 = 0.0;   // sum of the input theta over the owned cells
+    double S0 = 0.0;   // sum of the input theta over the lane's cells
+    // Lanes 0 and 63 (strip halo) and lanes past column nx own no cells.
+    // Every lane accumulates its own statistics; the ones that own nothing
+    // are dropped once, before the wavefront reductions.
+    bool owner = false;
 
     if (task < a.ntasks) {
         const int band = task / a.nstrips, strip = task % a.nstrips;
@@ -163,6 +178,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         const int col4 = L0 + 4 * lane;
         const bool ld_ok = col4 >= 0 && col4 < nx;
         const bool own_lane = lane >= 1 && lane <= kWave - 2 && col4 < nx;
+        owner = own_lane;
         const int ya = band * a.R;
         const int yb = min(ya + a.R, ny);
         const int y_lo = max(ya - K, 0);
@@ -202,22 +218,17 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                 set_elem(L, laste, 0.f);
             }
         };
-        // Owned-cell view of a row for the reductions: non-owned lanes and
-        // columns >= nx drop out (-inf for the max, 0 for the sums).
+        // A row's contribution to the lane's sums; on the lane holding column
+        // nx-1, columns >= nx drop out.
         auto row_sum = [&](float4 v) {
             if (ragged && lane == lastl) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
                     if (e > laste) elem(v, e) = 0.f;
             }
-            const float s = (v.x + v.y) + (v.z + v.w);
-            return own_lane ? s : 0.f;
+            return (v.x + v.y) + (v.z + v.w);
         };
-
-        auto own_sum = [&](float4 v) {
-            const float t = (v.x + v.y) + (v.z + v.w);
-            return own_lane ? t : 0.f;
-        };
+        auto own_sum = [&](float4 v) { return (v.x + v.y) + (v.z + v.w); };
 
         auto load_row = [&](int i) {
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -264,21 +275,21 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         };
 
         // One level-j row from level j-1's window.  GEN: any row of any strip
-        // (edge columns, edge rows, band-halo rows, mirror rows); otherwise
-        // an interior strip with every level on an owned interior row.
+        // (edge columns, edge rows, mirror rows); otherwise an interior strip
+        // whose levels touch neither the first two nor the last two rows.
         auto level = [&](auto PHC, auto JC, auto GENC, int i) {
             constexpr int j = decltype(JC)::value;
             constexpr bool GEN = decltype(GENC)::value;
             using JP = std::integral_constant<int, j - 1>;
             const int r = i - j;
-            if (GEN) {
-                const int lo = max(ya - (K - j), 0);
-                const int hi = min(yb + (K - j), ny);
-                if (r < lo || r >= hi) {
-                    if constexpr (j < K)
-                        if (r == ny) win_new(PHC, JC) = win_mid(PHC, JC);   // row ny := row ny-1
-                    return;
-                }
+            // level j is needed on rows [lo, hi): the band and the halo the
+            // remaining K - j levels consume (wave-uniform test)
+            const int lo = max(ya - (K - j), 0);
+            const int hi = min(yb + (K - j), ny);
+            if (r < lo || r >= hi) {
+                if constexpr (j < K)
+                    if (GEN && r == ny) win_new(PHC, JC) = win_mid(PHC, JC);   // row ny := row ny-1
+                return;
             }
             const float4 c = win_mid(PHC, JP{});
             const float4 up = win_old(PHC, JP{});
@@ -290,6 +301,8 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             const float xp[4] = {c.y, c.z, c.w, wr};
             const float uv[4] = {up.x, up.y, up.z, up.w};
             const float dv[4] = {dn.x, dn.y, dn.z, dn.w};
+            // theta' = kc*theta - L + S + cy*(up + down) + cx*(left + right),
+            // the last term added after the source
             float out4[4], loss4[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -297,14 +310,13 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                 float L;
                 if (RAD) L = fmaf(fmaf(fmaf(fmaf(a.a4, th, a.a3), th, a.a2), th, a.a1), th, a.a0);
                 else L = fmaf(a.a1, th, a.a0);
-                float d = fmaf(a.cx, fmaf(-2.f, th, xm[e] + xp[e]), -L);
-                d = fmaf(a.cy, fmaf(-2.f, th, uv[e] + dv[e]), d);
-                out4[e] = d;
+                out4[e] = fmaf(a.cy, uv[e] + dv[e], fmaf(a.kc, th, -L));
                 loss4[e] = L;
             }
             // source: the whole row is skipped unless it is near the laser
             float ssum = 0.f;
-            if (r >= srlo[j - 1] && r <= srhi[j - 1]) {
+            const bool src_row = r >= srlo[j - 1] && r <= srhi[j - 1];
+            if (src_row) {
                 const float uy = static_cast<float>(r) - yc[j - 1];
                 const float ey = a.ay * uy * uy;
 #pragma unroll
@@ -316,10 +328,10 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                     out4[e] += sv;
                     ssum += (col4 + e < nx) ? sv : 0.f;
                 }
-                ssum = own_lane ? ssum : 0.f;
             }
-            float4 res = make_float4(cv[0] + out4[0], cv[1] + out4[1], cv[2] + out4[2],
-                                     cv[3] + out4[3]);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) out4[e] = fmaf(a.cx, xm[e] + xp[e], out4[e]);
+            float4 res = make_float4(out4[0], out4[1], out4[2], out4[3]);
             float4 L4 = make_float4(loss4[0], loss4[1], loss4[2], loss4[3]);
             if (GEN) {
                 if (BC == LPBF_BC_DIRICHLET) {
@@ -334,7 +346,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                     fix_mirror(res);
                 }
             }
-            if (!GEN || (r >= ya && r < yb)) {
+            if (r >= ya && r < yb) {
                 if (j == K && own_lane)
                     *reinterpret_cast<float4*>((BUFS ? dst : a.out) + static_cast<long long>(r) * a.ld + col4) = res;
                 LevelAcc& A = acc[j - 1];
@@ -344,8 +356,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                     for (int e = 0; e < 4; ++e)
                         if (e > laste) elem(mx, e) = kNegInf;
                 }
-                float m4 = fmaxf(fmaxf(mx.x, mx.y), fmaxf(mx.z, mx.w));
-                m4 = own_lane ? m4 : kNegInf;
+                const float m4 = max4(mx);
                 if (m4 > A.bv) {
                     A.bv = m4;
                     const int e = (mx.x == m4) ? 0 : (mx.y == m4) ? 1 : (mx.z == m4) ? 2 : 3;
@@ -359,7 +370,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                 }
                 A.S += static_cast<double>(GEN ? row_sum(res) : own_sum(res));
                 A.SL += GEN ? row_sum(L4) : own_sum(L4);
-                A.SS += ssum;
+                if (src_row) A.SS += ssum;
                 // keep each running sum updated here: left alone the compiler
                 // defers the serial adds to the end of the unrolled trip and
                 // holds every phase's partial live (about 60 extra VGPRs)
@@ -412,11 +423,15 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             for (int q = 0; q + 1 < K; ++q) { W[q][1] = W[q][2]; W[q][2] = W[q][0]; }
         };
 
-        // steady state: interior strip, every level on an owned row that is
-        // neither an edge row nor next to one
+        // Fast rows: interior strip, and no level on row 0 or ny-1 (mirror
+        // and Dirichlet rows) or next to them.  A band clear of both domain
+        // edges runs fast from its first halo row, in six-row trips that may
+        // pass i_end: those rows load clamped rows and fail every level's
+        // row test.
         const bool interior = !has_left && !has_right && !has_last;
-        const int i_s = interior ? max(ya + K, K + 1) : i_end + 1;
-        const int i_e = interior ? min(yb + 1, ny - 1) : i_end + 1;
+        const bool clear = interior && ya >= K + 1 && yb + K <= ny - 2;
+        const int i_s = clear ? y_lo : interior ? max(ya + K, K + 1) : i_end + 1;
+        const int i_e = clear ? i_end + 6 : interior ? min(yb + 1, ny - 1) : i_end + 1;
         int i = y_lo;
         for (; i <= i_end && i < i_s; ++i) general(i);
         // scheduling barriers keep the phases apart: interleaving them buys
@@ -444,6 +459,11 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
     __shared__ double s_sum[4][kWaves];
     __shared__ long long s_m[kWaves];
     const long long N = static_cast<long long>(nx) * ny;
+    if (!owner) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j] = LevelAcc{kNegInf, 0x7fffffff, ny, -1, 0, 0.0, 0.f, 0.f};
+        S0 = 0.0;
+    }
     double prevS = wave_sum(S0);   // lane 0 holds the wavefront sums
 #pragma unroll
     for (int j = 0; j < K; ++j) {
@@ -528,6 +548,7 @@ inline F32Args make_args(const float* in, float* out, int ld, const StepConst& k
     const double dt = k.dt;
     a.cx = static_cast<float>(k.alpha * dt * k.inv_dx2);
     a.cy = static_cast<float>(k.alpha * dt * k.inv_dy2);
+    a.kc = static_cast<float>(1.0 - 2.0 * k.alpha * dt * (k.inv_dx2 + k.inv_dy2));
     // dt*loss in K as a polynomial in theta = T - T_ref:
     //   cH*(theta + T_ref - Tamb) + cB*(theta + T_ref - Tsub)
     //   + cR*((T_ref + theta)^4 - Tamb^4)
