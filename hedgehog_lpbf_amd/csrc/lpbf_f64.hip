@@ -31,6 +31,7 @@ constexpr double kSigmaSB = 5.670374419e-8;  // params.hpp:11
 constexpr int kBlockCols = 128;              // columns per wavefront
 constexpr int kThreads = 256;                // 4 wavefronts
 constexpr int kRowsPerWave = 32;
+constexpr int kAhead = 3;                    // rows in flight ahead of the one computed
 
 // IN718 Mills tables (params.hpp:67-73) and their interpolation (:75-111).
 __constant__ double cT_PTS[9] = {298, 473, 673, 873, 1073, 1273, 1473, 1533, 1609};
@@ -226,25 +227,39 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
             }
         }
 
-        // row y+1 is fetched one iteration ahead, so its load overlaps the
-        // arithmetic of row y (the next row's values are only moved, never
-        // recomputed: the cell arithmetic is unchanged)
-        double f0 = 0.0, f1 = 0.0, fe = 0.0;
-        if (ya < yb && (ya + 1 < ny || tile)) {
-            const double* rn = g.in + static_cast<long long>(ya + 1) * nx;
-            f0 = a0 ? rn[x0] : 0.0;
-            f1 = a1 ? rn[x1] : 0.0;
-            fe = eok ? rn[ecol] : 0.0;
+        // rows y+1 .. y+kAhead are in flight while row y is computed (the
+        // values are only moved, never recomputed: the cell arithmetic is
+        // unchanged); a row r is fetched when it is consumed (r <= yb) and
+        // exists (r < ny, or the tile's halo row)
+        const int r_last = (yb < ny || tile) ? yb : ny - 1;
+        double q0[kAhead], q1[kAhead], qe[kAhead];
+#pragma unroll
+        for (int d = 0; d < kAhead; ++d) {
+            const int r = ya + 1 + d;
+            q0[d] = q1[d] = qe[d] = 0.0;
+            if (r <= r_last) {
+                const double* rn = g.in + static_cast<long long>(r) * nx;
+                q0[d] = a0 ? rn[x0] : 0.0;
+                q1[d] = a1 ? rn[x1] : 0.0;
+                qe[d] = eok ? rn[ecol] : 0.0;
+            }
         }
         for (int y = ya; y < yb; ++y) {
             const bool has_next = (y + 1 < ny || tile);
-            const double n0v = has_next ? f0 : c0v, n1v = has_next ? f1 : c1v;
-            const double ne = has_next ? fe : ce;
-            if (y + 1 < yb && (y + 2 < ny || tile)) {
-                const double* rn = g.in + static_cast<long long>(y + 2) * nx;
-                f0 = a0 ? rn[x0] : 0.0;
-                f1 = a1 ? rn[x1] : 0.0;
-                fe = eok ? rn[ecol] : 0.0;
+            const double n0v = has_next ? q0[0] : c0v, n1v = has_next ? q1[0] : c1v;
+            const double ne = has_next ? qe[0] : ce;
+#pragma unroll
+            for (int d = 0; d + 1 < kAhead; ++d) {
+                q0[d] = q0[d + 1];
+                q1[d] = q1[d + 1];
+                qe[d] = qe[d + 1];
+            }
+            const int r = y + 1 + kAhead;
+            if (r <= r_last) {
+                const double* rn = g.in + static_cast<long long>(r) * nx;
+                q0[kAhead - 1] = a0 ? rn[x0] : 0.0;
+                q1[kAhead - 1] = a1 ? rn[x1] : 0.0;
+                qe[kAhead - 1] = eok ? rn[ecol] : 0.0;
             }
             const double up0 = lane_from_left(c0v), up1 = lane_from_left(c1v);
             const double dn0 = lane_from_right(c0v), dn1 = lane_from_right(c1v);
