@@ -48,6 +48,30 @@ def test_step_single_parity(ora, nx, ny, phys):
     assert np.abs(got - ref).max() <= TOL
 
 
+@pytest.mark.parametrize("phys", [0, 1])
+def test_step_bitwise_without_exp(ora, phys):
+    """With the beam far outside the plate no cell calls exp, the one libm
+    function the device and glibc may round differently, so the step is
+    bit-identical to the oracle.  Pins the kernel's divisions (reciprocal
+    plus one remainder correction), the property tables over their whole
+    range (300..2600 K, the mushy zone included) and the row skip of the
+    source."""
+    print(f"\n[case] step bitwise without exp phys={phys}", flush=True)
+    nx, ny = 300, 133
+    s = _case(ora, nx, ny)
+    k = ora.make_const(s, ora.dt_cfl(s) if phys == 0 else ora.dt_cfl_ext(s, 1.8))
+    k.kmult = 1.8 if phys else 1.0
+    k.x_l, k.y_l = 0.5 * s.dom.Lx, 40.0 * s.dom.Ly
+    T = _rand_field(ny, nx, 1234 + phys)
+    ref = ora.step(T, k, phys)
+    a, b = dev(T), dev(np.zeros_like(T))
+    h.lib().lpbf_launch_step(a.data_ptr(), b.data_ptr(), kconst(k), phys, stream())
+    sync_check()
+    got = b.cpu().numpy()
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), \
+        int((got != ref).sum())
+
+
 def test_case0_full_run_fused_path(ora):
     """The reference GpuStepTask fused mode (gpu_step_task.hpp:149-186):
     step_reduce + device finalize + hot-column extraction every step, run
