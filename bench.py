@@ -78,7 +78,7 @@ def cpu_baseline(k, nx, ny, target_s=12.0):
                 break
     return {"value": nx * ny * steps / el, "unit": "cell-updates/s", "cores": nt,
             "kind": "port",
-            "sample": f"oracle FP64 step (C, -O2) on {nt} host threads (row ranges), "
+            "sample": f"oracle FP64 step (C, -O3) on {nt} host threads (row ranges), "
                       f"{nx}x{ny} plate, {steps} steps, {el:.1f} s"}
 
 
@@ -188,7 +188,7 @@ def batch_bench(args):
                     break
             el = time.perf_counter() - t0
         cpu = {"value": done / el, "unit": "cell-updates/s", "cores": nt, "kind": "port",
-               "sample": f"oracle FP64 step (C, -O2) on {nt} host threads, first {n_inst} "
+               "sample": f"oracle FP64 step (C, -O3) on {nt} host threads, first {n_inst} "
                          f"instances x 10 steps, {el:.1f} s"}
     achieved = 8.0 * cells * steps / (total_ms * 1e-3) / 1e9
     out = {
