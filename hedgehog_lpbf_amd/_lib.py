@@ -107,6 +107,7 @@ SIGNATURES = {
     "lpbf_launch_step_ex_plan": (_v, [_vp, _vp, _KP, _i, _i, _d, _d, _vp, _vp, _vp, _vp, _vp,
                                       _vp, _i, _vp, _vp]),
     "lpbf_launch_noise_f64": (_v, [_vp, _ll, _ull, _d, _d, _vp]),
+    "lpbf_probe_exp_f64": (_v, [_vp, _vp, _ll, _vp]),
     # FP32 path
     "lpbf_f32_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "lpbf_launch_step_f32": (_v, [_vp, _vp, _i, _KP, _d, _i, _d, _d, _vp, _i, _i, _i,

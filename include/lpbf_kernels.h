@@ -115,6 +115,13 @@ void lpbf_launch_stats_finalize(const StepStats* parts, int nblocks, int nx,
 void lpbf_launch_noise_f64(double* T, long long n, unsigned long long seed,
                            double base, double amp, hipStream_t stream);
 
+/* y[i] = exp(x[i]) with the device exp the FP64 step kernels call, n values.
+ * No reference counterpart: the reference step calls libm exp on the CPU
+ * (include/step_task.hpp:113, 168) and the CUDA device exp on the GPU
+ * (src/gpu/kernels.cu:115); this exposes the one libm function at the
+ * parity boundary so its distance to a host libm can be measured. */
+void lpbf_probe_exp_f64(const double* x, double* y, long long n, hipStream_t stream);
+
 /* ---- FP32 path (no reference counterpart) ---------------------------------
  *
  * Field layout: theta = T - T_ref in FP32, row y at theta + y*ld, ld >= nx,
