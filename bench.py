@@ -7,7 +7,9 @@ Workload (default, `--workload plate8k_f32`): an 8192 x 8192 plate, dx = dy
 steps are its first K), mirror edges, FP32 field (theta = T - 300 K) with the
 fused per-step statistics (peak T + index, molten rows and count, energy
 sums) in FP64 across the grid.  One bench step = one time step of the whole
-plate.  `--workload plate8k_f64` runs the same plate on the FP64 parity path;
+plate.  `--workload plate8k_f64` runs the same plate on the FP64 parity path
+(temporally blocked, bitwise the single-step field; depth and band height
+autotuned);
 `batch256` steps 256 process-parameter instances of uneven size together
 (one bench step = one time step of every instance); `case0` times the
 reference's published case on the device-resident FP64 path.
@@ -15,7 +17,8 @@ reference's published case on the device-resident FP64 path.
 Prints one JSON line: value = cell-updates/s over the timed steps, with the
 inputs already resident in HBM; `roofline` for the dominant kernel (step
 kernel duration from HIP events on its stream); `cpu_baseline` = the CPU
-oracle (FP64, one thread) timed on a bounded sample of the same plate.
+oracle (FP64, all host threads on row ranges) timed on a bounded sample of
+the same plate.
 """
 from __future__ import annotations
 
@@ -387,9 +390,10 @@ def main():
                          "plain device copy of one n x n FP32 plate per step, the reference "
                          "for calibrating HBM byte counters")
     ap.add_argument("--depth", type=int, default=0,
-                    help="FP32 steps per launch (1-4); 0 = pick depth and band rows with "
-                         "the library's autotuner before the warm-up")
-    ap.add_argument("--rows", type=int, default=0, help="FP32 band height (0 = built-in)")
+                    help="plate workloads: steps per launch (1-4); 0 = pick depth and band "
+                         "rows with the library's autotuner before the warm-up")
+    ap.add_argument("--rows", type=int, default=0,
+                    help="plate workloads: band height (0 = built-in)")
     ap.add_argument("--n", type=int, default=8192, help="plate side (cells)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -464,25 +468,39 @@ def main():
                                                stats.data_ptr() + 56 * s, st)
         kernel_name = "k_step_f32"
     else:
-        depth = 1
+        # FP64 parity path with temporal blocking: the field is bitwise the
+        # one `depth` single steps give
+        depth, rows = args.depth, args.rows
+        if depth == 0:
+            bd, br, bms = C.c_int(), C.c_int(), C.c_double()
+            t0 = time.perf_counter()
+            rc = lib.lpbf_f64_blocked_autotune(k, 0, 0, 300.0, mat.Tm, pdev.data_ptr(),
+                                               plan.shape[0], 4, st, C.byref(bd), C.byref(br),
+                                               C.byref(bms))
+            if rc != 0:
+                h.check()
+                raise RuntimeError(f"autotune failed: {rc}")
+            depth, rows = bd.value, br.value
+            tuned = {"depth": depth, "rows": rows, "ms_per_step": bms.value,
+                     "tune_s": time.perf_counter() - t0}
+            print(f"[bench] autotune: depth {depth} rows {rows} ({bms.value:.4f} ms/step)",
+                  file=sys.stderr, flush=True)
         a = torch.empty((ny, nx), dtype=torch.float64, device=dev)
         b = torch.empty_like(a)
         lib.lpbf_launch_noise_f64(a.data_ptr(), cells, 0, 300.0, 0.5, st)
-        nb = lib.lpbf_step_ex_nblocks(nx, ny)
-        work = torch.empty(nb * 7 + 7, dtype=torch.float64, device=dev)
+        work = torch.empty(lib.lpbf_f64_blocked_workspace_bytes(nx, ny, depth, rows) // 8 + 8,
+                           dtype=torch.float64, device=dev)
         bytes_per_update = 16
-        kk = h.StepConst.from_buffer_copy(bytes(k))
 
         def launch(s, d, bufs):
             x, y = bufs
-            kk.x_l, kk.y_l, kk.q_scale = plan[s]
-            lib.lpbf_launch_step_ex(x.data_ptr(), y.data_ptr(), kk, 0, 0, 300.0,
-                                    mat.Tm, work.data_ptr(), st)
+            lib.lpbf_launch_step_f64_blocked(x.data_ptr(), y.data_ptr(), k, 0, 0, 300.0, mat.Tm,
+                                             pdev.data_ptr(), s, d, rows,
+                                             stats.data_ptr() + 56 * s, work.data_ptr(), st)
 
         def finalize(s, d):
-            lib.lpbf_launch_stats_finalize(work.data_ptr(), nb, nx, ny, stats.data_ptr() + 56 * s,
-                                           st)
-        kernel_name = "k_step_f64"
+            pass                    # the blocked launch merges its statistics itself
+        kernel_name = "k_step_f64_tb"
 
     bufs = [a, b]
 
@@ -548,10 +566,10 @@ def main():
         "config": {
             "workload": f"{nx}x{ny} plate, dx 5 um, dt 1.2 us, 20-track raster P 250 W "
                         f"V 1.2 m/s, T0 300 K +/-0.5 K noise, "
-                        f"{'FP32 theta field' if depth > 1 or args.workload == 'plate8k_f32' else 'FP64 field'}"
+                        f"{'FP32 theta field' if args.workload == 'plate8k_f32' else 'FP64 field'}"
                         f", fused per-step stats",
             "nx": nx, "ny": ny, "temporal_depth": depth,
-            "band_rows": rows if args.workload == "plate8k_f32" else None,
+            "band_rows": rows,
             "autotuned": tuned,
         },
         "roofline": {

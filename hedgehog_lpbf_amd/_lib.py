@@ -108,6 +108,11 @@ SIGNATURES = {
                                       _vp, _i, _vp, _vp]),
     "lpbf_launch_noise_f64": (_v, [_vp, _ll, _ull, _d, _d, _vp]),
     "lpbf_probe_exp_f64": (_v, [_vp, _vp, _ll, _vp]),
+    "lpbf_f64_blocked_workspace_bytes": (C.c_size_t, [_i, _i, _i, _i]),
+    "lpbf_launch_step_f64_blocked": (_v, [_vp, _vp, _KP, _i, _i, _d, _d, _vp, _i, _i, _i, _vp,
+                                          _vp, _vp]),
+    "lpbf_f64_blocked_autotune": (_i, [_KP, _i, _i, _d, _d, _vp, _i, _i, _vp, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     # FP32 path
     "lpbf_f32_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "lpbf_launch_step_f32": (_v, [_vp, _vp, _i, _KP, _d, _i, _d, _d, _vp, _i, _i, _i,

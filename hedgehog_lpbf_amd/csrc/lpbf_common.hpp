@@ -12,6 +12,16 @@
 namespace lpbf_amd {
 
 constexpr int kWave = 64;
+constexpr int kXcds = 8;    // accelerator complex dies of an MI355X
+
+// Workgroup ids are dealt to the XCDs round-robin (id % 8); this maps them so
+// that each XCD runs a contiguous range of tiles, which then share halo
+// cache lines in that XCD's L2.
+__device__ inline int xcd_remap(int bid, int n) {
+    const int per = n / kXcds, rem = n % kXcds;
+    const int xcd = bid % kXcds, local = bid / kXcds;
+    return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
 
 // Records the first error since the last lpbf_get_last_error() query.
 void record_error(int err);

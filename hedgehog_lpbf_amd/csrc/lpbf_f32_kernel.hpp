@@ -43,7 +43,6 @@ constexpr int kOv = 4;                          // overlap columns per side
 constexpr int kLoadCols = 4 * kWave;            // 256
 constexpr int kOutCols = kLoadCols - 2 * kOv;   // 248
 constexpr int kMaxDepth = 4;
-constexpr int kXcds = 8;
 constexpr int kPf = 2;                          // row prefetch distance
 
 struct F32Args {
@@ -75,12 +74,6 @@ __device__ __forceinline__ float max4(float4 v) {
     asm("v_max3_f32 %0, %1, %2, %3" : "=v"(m) : "v"(v.x), "v"(v.y), "v"(v.z));
     asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(m), "v"(v.w));
     return r;
-}
-
-__device__ inline int xcd_remap(int bid, int n) {
-    const int per = n / kXcds, rem = n % kXcds;
-    const int xcd = bid % kXcds, local = bid / kXcds;
-    return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
 // Whole-wavefront lane shifts (DPP wave_shr:1 / wave_shl:1, GFX9 family):

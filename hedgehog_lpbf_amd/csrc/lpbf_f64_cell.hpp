@@ -161,5 +161,23 @@ __device__ __forceinline__ double cell_ext(double T, double Txm, double Txp,
     return T + div_rn(k.dt * rhs, rcp, r_rcp);
 }
 
+// Whole-wavefront lane moves of a double without LDS: DPP wave_shr:1 /
+// wave_shl:1 on both halves (lane l receives lane l-1 / l+1; the lane
+// without a source receives 0), and a lane broadcast through readlane.
+__device__ __forceinline__ double lane_from_left(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_from_right(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_read(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
 }  // namespace
 }  // namespace lpbf_amd

@@ -110,6 +110,33 @@ void lpbf_launch_step_ex_plan(const double* Tin, double* Tout, const StepConst* 
 void lpbf_launch_stats_finalize(const StepStats* parts, int nblocks, int nx,
                                 int ny, StepStats* out, hipStream_t stream);
 
+/* FP64 steps with temporal blocking: one launch advances Tin by `depth`
+ * (1..4) steps into Tout, the laser of step step0+j taken from d_plan[step0+j]
+ * (device, d_plan must hold step0+depth entries; k's x_l, y_l, q_scale are
+ * ignored), bc LPBF_BC_MIRROR or LPBF_BC_DIRICHLET as lpbf_launch_step_ex.
+ * Every cell of every step evaluates the same expression as
+ * lpbf_launch_step_ex, so Tout is bitwise the field of `depth` such calls.
+ * rows: band height (0 = built-in).  If stats != NULL it receives `depth`
+ * StepStats (one per step, merged on the device; the FP64 sums are added in
+ * a different order than lpbf_launch_step_ex's, so they agree to rounding)
+ * and work must hold lpbf_f64_blocked_workspace_bytes(nx, ny, depth, rows)
+ * bytes of device memory. */
+size_t lpbf_f64_blocked_workspace_bytes(int nx, int ny, int depth, int rows);
+void lpbf_launch_step_f64_blocked(const double* Tin, double* Tout, const StepConst* k,
+                                  int phys, int bc, double T_bc, double Tm,
+                                  const LaserTick* d_plan, int step0, int depth, int rows,
+                                  StepStats* stats, void* work, hipStream_t stream);
+
+/* Autotuner for lpbf_launch_step_f64_blocked: times depths 1..max_depth and
+ * a set of band heights on scratch copies of a k->nx x k->ny field
+ * (allocated and freed here, synchronising `stream`) and returns the fastest
+ * per step.  d_plan needs max_depth entries.  Returns 0 or an error code
+ * (also recorded for lpbf_get_last_error). */
+int lpbf_f64_blocked_autotune(const StepConst* k, int phys, int bc, double T_bc, double Tm,
+                              const LaserTick* d_plan, int n_plan, int max_depth,
+                              hipStream_t stream, int* best_depth, int* best_rows,
+                              double* best_ms_per_step);
+
 /* Counter-based noise field: T[i] = base + 2*amp*u(seed, i), u in [-0.5,0.5)
  * from SplitMix64 with 24 random bits; n values. */
 void lpbf_launch_noise_f64(double* T, long long n, unsigned long long seed,
