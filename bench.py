@@ -579,8 +579,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": (traffic_for(args.workload, depth=depth, rows=rows)
-                        if args.workload == "plate8k_f32" else traffic_for(args.workload)),
+            "traffic": traffic_for(args.workload, depth=depth, rows=rows),
             "alg_bytes_per_launch": alg_bytes,
             "kernel_ms_per_launch": kern_ms,
         },
