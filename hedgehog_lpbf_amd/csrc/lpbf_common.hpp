@@ -5,7 +5,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
+#include <vector>
 
 #include "lpbf_step_const.h"
 
@@ -21,6 +23,26 @@ __device__ inline int xcd_remap(int bid, int n) {
     const int per = n / kXcds, rem = n % kXcds;
     const int xcd = bid % kXcds, local = bid / kXcds;
     return (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+// Band heights an autotuner times: a fixed ladder, plus for m = 1..6 rounds
+// of `slots` resident wavefronts the height whose band count puts the
+// wavefront count just under m whole rounds (a last round that is only
+// partly filled leaves SIMDs idle at the end of the launch).  0 stands for
+// the built-in height.  Heights below `min_rows` or above ny are dropped.
+inline std::vector<int> band_candidates(int ny, long long nstrips, long long slots,
+                                        std::initializer_list<int> ladder, int min_rows) {
+    std::vector<int> out;
+    for (int R : ladder)
+        if (R == 0 || (R >= min_rows && R <= ny)) out.push_back(R);
+    for (long long m = 1; m <= 6 && nstrips > 0; ++m) {
+        const long long bands = std::max(1LL, m * slots / nstrips);
+        const int R = static_cast<int>((ny + bands - 1) / bands);
+        if (R >= min_rows && R <= ny) out.push_back(R);
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
 }
 
 // Records the first error since the last lpbf_get_last_error() query.

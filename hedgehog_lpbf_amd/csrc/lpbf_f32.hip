@@ -8,7 +8,7 @@ namespace lpbf_amd {
 namespace {
 
 template <int K, int BC, bool RAD>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_step_f32(F32Args a,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == 4 && !RAD ? 2 : 3))) void k_step_f32(F32Args a,
                                                        const LaserTick* __restrict__ plan,
                                                        int step0) {
     step_tile<K, BC, RAD>(a, nullptr, nullptr, plan, step0, xcd_remap(blockIdx.x, a.nwg));
@@ -149,11 +149,13 @@ int lpbf_f32_autotune(const StepConst* k, double T_ref, int bc, double T_bc, dou
         return LPBF_ERR_INVALID_ARGUMENT;
     }
     const int nx = k->nx, ny = k->ny, ld = (nx + 3) & ~3;
-    static const int kRows[] = {0, 16, 24, 32, 40, 46, 56, 64, 92, 128, 192};
+    // three resident waves per SIMD at depth 4, 1024 SIMDs
+    const std::vector<int> rows_c = band_candidates(
+        ny, (nx + kOutCols - 1) / kOutCols, 3 * 1024, {0, 16, 24, 32, 40, 48, 56, 64, 96, 128, 192},
+        1);
     size_t work_bytes = 0;
     for (int d = 1; d <= max_depth; ++d)
-        for (int R : kRows) {
-            if (R > ny) continue;
+        for (int R : rows_c) {
             const size_t b = lpbf_f32_workspace_bytes(nx, ny, d, R);
             if (b > work_bytes) work_bytes = b;
         }
@@ -171,11 +173,11 @@ int lpbf_f32_autotune(const StepConst* k, double T_ref, int bc, double T_bc, dou
     int bd = 1, br = 0;
     if (err == hipSuccess) {
         lpbf_launch_noise_f32(a, nx, ny, ld, 1, 0.0, 0.5, stream);
-        const int reps = 3;
+        const int reps = 10;   // long enough that launch-to-launch noise does not pick the tiling
         const int d_first = (max_depth == 1) ? 1 : 2;
         for (int depth = d_first; depth <= max_depth && err == hipSuccess; ++depth) {
-            for (int R : kRows) {
-                if (R > ny || (R > 0 && R < 2 * depth)) continue;
+            for (int R : rows_c) {
+                if (R > 0 && R < 2 * depth) continue;
                 lpbf_launch_step_f32(a, b, ld, k, T_ref, bc, T_bc, Tm, d_plan, 0, depth, R, nullptr,
                                      work, stream);   // warm-up
                 err = hipEventRecord(e0, stream);

@@ -186,6 +186,8 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         const int lastl = pL >> 2, laste = pL & 3;
         // the lane that holds column nx-1 may also hold columns >= nx
         const bool ragged = has_right && pade != 0;
+        // strips holding a domain edge column take the column fix-ups
+        const bool edge = has_left || has_right || has_last;
 
         // Mirror: column -1 := column 0, column nx := column nx-1.
         auto fix_mirror = [&](float4& v) {
@@ -221,7 +223,6 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             }
             return (v.x + v.y) + (v.z + v.w);
         };
-        auto own_sum = [&](float4 v) { return (v.x + v.y) + (v.z + v.w); };
 
         auto load_row = [&](int i) {
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -229,11 +230,14 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                 v = *reinterpret_cast<const float4*>((BUFS ? src : a.in) + static_cast<long long>(i) * a.ld + col4);
             return v;
         };
-        // interior strips only (every lane in bounds): rows past y_hi are
-        // never consumed, so the row index is clamped instead of tested
-        auto load_row_interior = [&](int i) {
+        // rows past y_hi are never consumed, so the row index is clamped
+        // instead of tested; lanes outside [0, ld) load a clamped column (the
+        // values are never owned, and the edge fix-ups overwrite the ghost
+        // columns the owned cells read)
+        const int col4c = min(max(col4, 0), a.ld - 4);
+        auto load_row_clamped = [&](int i) {
             const int ii = min(i, y_hi - 1);
-            return *reinterpret_cast<const float4*>((BUFS ? src : a.in) + static_cast<long long>(ii) * a.ld + col4);
+            return *reinterpret_cast<const float4*>((BUFS ? src : a.in) + static_cast<long long>(ii) * a.ld + col4c);
         };
 
         // Register rings, indexed by a compile-time phase so the window
@@ -267,9 +271,9 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             else return W[j - 1][(ph + 1) % 3];
         };
 
-        // One level-j row from level j-1's window.  GEN: any row of any strip
-        // (edge columns, edge rows, mirror rows); otherwise an interior strip
-        // whose levels touch neither the first two nor the last two rows.
+        // One level-j row from level j-1's window.  GEN: any row (edge rows,
+        // mirror rows); otherwise a band whose levels touch neither the first
+        // two nor the last two rows.  Edge columns are fixed up in both.
         auto level = [&](auto PHC, auto JC, auto GENC, int i) {
             constexpr int j = decltype(JC)::value;
             constexpr bool GEN = decltype(GENC)::value;
@@ -326,25 +330,20 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             for (int e = 0; e < 4; ++e) out4[e] = fmaf(a.cx, xm[e] + xp[e], out4[e]);
             float4 res = make_float4(out4[0], out4[1], out4[2], out4[3]);
             float4 L4 = make_float4(loss4[0], loss4[1], loss4[2], loss4[3]);
-            if (GEN) {
-                if (BC == LPBF_BC_DIRICHLET) {
-                    if (r == 0 || r == ny - 1) {
-                        res = make_float4(a.theta_bc, a.theta_bc, a.theta_bc, a.theta_bc);
-                        L4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                        ssum = 0.f;
-                    } else {
-                        fix_dirichlet(res, L4);
-                    }
-                } else {
-                    fix_mirror(res);
-                }
+            if (GEN && BC == LPBF_BC_DIRICHLET && (r == 0 || r == ny - 1)) {
+                res = make_float4(a.theta_bc, a.theta_bc, a.theta_bc, a.theta_bc);
+                L4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                ssum = 0.f;
+            } else if (edge) {
+                if (BC == LPBF_BC_DIRICHLET) fix_dirichlet(res, L4);
+                else fix_mirror(res);
             }
             if (r >= ya && r < yb) {
                 if (j == K && own_lane)
                     *reinterpret_cast<float4*>((BUFS ? dst : a.out) + static_cast<long long>(r) * a.ld + col4) = res;
                 LevelAcc& A = acc[j - 1];
                 float4 mx = res;
-                if (GEN && ragged && lane == lastl) {
+                if (ragged && lane == lastl) {
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
                         if (e > laste) elem(mx, e) = kNegInf;
@@ -361,8 +360,8 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                     A.ymin = min(A.ymin, r);
                     A.ymax = r;
                 }
-                A.S += static_cast<double>(GEN ? row_sum(res) : own_sum(res));
-                A.SL += GEN ? row_sum(L4) : own_sum(L4);
+                A.S += static_cast<double>(row_sum(res));
+                A.SL += row_sum(L4);
                 if (src_row) A.SS += ssum;
                 // keep each running sum updated here: left alone the compiler
                 // defers the serial adds to the end of the unrolled trip and
@@ -381,12 +380,12 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         auto body = [&](auto PHC, auto GENC, int i) {
             constexpr int ph = decltype(PHC)::value;
             constexpr bool GEN = decltype(GENC)::value;
-            Z[(ph + 3) % 6] = GEN ? load_row(i + 3) : load_row_interior(i + 3);
+            Z[(ph + 3) % 6] = GEN ? load_row(i + 3) : load_row_clamped(i + 3);
             float4& v = win_new(PHC, J0{});
             if (!GEN || i < y_hi) {
-                if (GEN && BC == LPBF_BC_MIRROR) fix_mirror(v);
+                if (edge && BC == LPBF_BC_MIRROR) fix_mirror(v);
                 if (i >= ya && i < yb) {
-                    S0 += static_cast<double>(GEN ? row_sum(v) : own_sum(v));
+                    S0 += static_cast<double>(row_sum(v));
                     pin(S0);
                 }
             } else if (i == ny) {
@@ -416,15 +415,13 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             for (int q = 0; q + 1 < K; ++q) { W[q][1] = W[q][2]; W[q][2] = W[q][0]; }
         };
 
-        // Fast rows: interior strip, and no level on row 0 or ny-1 (mirror
-        // and Dirichlet rows) or next to them.  A band clear of both domain
-        // edges runs fast from its first halo row, in six-row trips that may
-        // pass i_end: those rows load clamped rows and fail every level's
-        // row test.
-        const bool interior = !has_left && !has_right && !has_last;
-        const bool clear = interior && ya >= K + 1 && yb + K <= ny - 2;
-        const int i_s = clear ? y_lo : interior ? max(ya + K, K + 1) : i_end + 1;
-        const int i_e = clear ? i_end + 6 : interior ? min(yb + 1, ny - 1) : i_end + 1;
+        // Fast rows: no level on row 0 or ny-1 (mirror and Dirichlet rows)
+        // or next to them.  A band clear of both domain edges runs fast from
+        // its first halo row, in six-row trips that may pass i_end: those
+        // rows load clamped rows and fail every level's row test.
+        const bool clear = ya >= K + 1 && yb + K <= ny - 2;
+        const int i_s = clear ? y_lo : max(ya + K, K + 1);
+        const int i_e = clear ? i_end + 6 : min(yb + 1, ny - 1);
         int i = y_lo;
         for (; i <= i_end && i < i_s; ++i) general(i);
         // scheduling barriers keep the phases apart: interleaving them buys

@@ -401,10 +401,15 @@ int lpbf_f64_blocked_autotune(const StepConst* k, int phys, int bc, double T_bc,
         return LPBF_ERR_INVALID_ARGUMENT;
     }
     const int nx = k->nx, ny = k->ny;
-    static const int kRows[] = {0, 16, 32, 48, 64, 96};
+    // three resident waves per SIMD up to depth 3, 1024 SIMDs
+    auto rows_for = [&](int depth) {
+        const int out = kTbCols - 4 * ((depth + 1) / 2);
+        return band_candidates(ny, (nx + out - 1) / out, 3 * 1024, {0, 16, 32, 48, 64, 96},
+                               2 * depth);
+    };
     size_t work_bytes = 0;
     for (int d = 1; d <= max_depth; ++d)
-        for (int R : kRows) {
+        for (int R : rows_for(d)) {
             const size_t b = lpbf_f64_blocked_workspace_bytes(nx, ny, d, R);
             if (b > work_bytes) work_bytes = b;
         }
@@ -425,10 +430,9 @@ int lpbf_f64_blocked_autotune(const StepConst* k, int phys, int bc, double T_bc,
     int bd = 1, br = 0;
     if (err == hipSuccess) {
         lpbf_launch_noise_f64(a, static_cast<long long>(nx) * ny, 1, 300.0, 0.5, stream);
-        const int reps = 3;
+        const int reps = 10;   // long enough that launch-to-launch noise does not pick the tiling
         for (int depth = 1; depth <= max_depth && err == hipSuccess; ++depth) {
-            for (int R : kRows) {
-                if (R > ny) continue;
+            for (int R : rows_for(depth)) {
                 lpbf_launch_step_f64_blocked(a, b, k, phys, bc, T_bc, Tm, d_plan, 0, depth, R,
                                              st, work, stream);   // warm-up
                 err = hipEventRecord(e0, stream);
