@@ -466,10 +466,10 @@ int lpbf_f32_batch_autotune(lpbf_f32_batch* b, int max_depth, hipStream_t stream
                 for (int R : kRows) {
                     if (R > 0 && (R < 2 * d || R > min_ny)) continue;
                     double ms_sum = 0.0;
+                    const int reps = 6;   // enough launches that timing noise does not pick
                     for (auto const& m : mem) {
                         Batch::Group g = B.build(m.second, d, m.first, R, false);
                         try {
-                            const int reps = 3;
                             B.launch(g, d, 0, stream);   // warm-up
                             lpbf_amd::ok(hipEventRecord(e0, stream));
                             for (int q = 0; q < reps; ++q) B.launch(g, d, 0, stream);
@@ -481,7 +481,7 @@ int lpbf_f32_batch_autotune(lpbf_f32_batch* b, int max_depth, hipStream_t stream
                         }
                         float ms = 0.f;
                         lpbf_amd::ok(hipEventElapsedTime(&ms, e0, e1));
-                        ms_sum += ms / (3.0 * d);
+                        ms_sum += ms / (static_cast<double>(reps) * d);
                         Batch::free_group(g);
                     }
                     if (ms_sum < best) { best = ms_sum; bd = d; br = R; }
