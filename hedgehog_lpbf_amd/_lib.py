@@ -108,6 +108,10 @@ SIGNATURES = {
                                       _vp, _i, _vp, _vp]),
     "lpbf_launch_noise_f64": (_v, [_vp, _ll, _ull, _d, _d, _vp]),
     "lpbf_probe_exp_f64": (_v, [_vp, _vp, _ll, _vp]),
+    "lpbf_step_ex_batch_nblocks": (_i, [_i, _i]),
+    "lpbf_launch_step_ex_batch": (_v, [_vp, _vp, _KP, _i, _i, _d, _d, _vp, _i, _vp, _vp, _vp, _vp,
+                                       _vp]),
+    "lpbf_launch_step_ex_batch_flush": (_v, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "lpbf_f64_blocked_workspace_bytes": (C.c_size_t, [_i, _i, _i, _i]),
     "lpbf_launch_step_f64_blocked": (_v, [_vp, _vp, _KP, _i, _i, _d, _d, _vp, _i, _i, _i, _vp,
                                           _vp, _vp]),

@@ -106,18 +106,17 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
                 hip_ok(hipMemcpyAsync(d_cruise_, cr.data(), cr.size(), hipMemcpyHostToDevice,
                                       S(stream_)), "cruise upload");
                 hip_ok(hipStreamSynchronize(S(stream_)), "cruise upload");
-                hip_ok(hipMalloc(reinterpret_cast<void**>(&d_step_), 2 * sizeof(int)),
-                       "step counter and ticket");
-                hip_ok(hipMemsetAsync(d_step_, 0, 2 * sizeof(int), S(stream_)), "ticket");
-                const int pb = lpbf_step_ex_plan_nblocks(nx_, ny_);
+                hip_ok(hipMalloc(reinterpret_cast<void**>(&d_bticks_), B * sizeof(LaserTick)),
+                       "batch ticks");
+                hip_ok(hipMalloc(reinterpret_cast<void**>(&d_bcruise_), B), "batch cruise flags");
+                // the batch launches keep two steps' block records
+                const int pb = 2 * lpbf_step_ex_batch_nblocks(nx_, ny_);
                 if (pb > nblk_) {
                     hip_ok(hipFree(d_blk_), "block records");
                     d_blk_ = nullptr;
                     hip_ok(hipMalloc(&d_blk_, static_cast<std::size_t>(pb) * sizeof(StepStats)),
                            "block records");
                 }
-                hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_step_), sizeof(int)),
-                       "pinned step");
             }
         } else {
             ld_ = (nx_ + 3) & ~3;
@@ -167,12 +166,12 @@ void DeviceRun::release_locked() noexcept {
         if (graph_[q]) (void)hipGraphDestroy(static_cast<hipGraph_t>(graph_[q]));
         gexec_[q] = graph_[q] = nullptr;
     }
-    if (d_step_) (void)hipFree(d_step_);
+    if (d_bticks_) (void)hipFree(d_bticks_);
+    if (d_bcruise_) (void)hipFree(d_bcruise_);
     if (d_cruise_) (void)hipFree(d_cruise_);
-    if (h_step_) (void)hipHostFree(h_step_);
-    d_step_ = nullptr;
+    d_bticks_ = nullptr;
+    d_bcruise_ = nullptr;
     d_cruise_ = nullptr;
-    h_step_ = nullptr;
     for (void* p : {d_a_, d_b_, d_blk_, static_cast<void*>(d_plan_), static_cast<void*>(d_ring_),
                     static_cast<void*>(d_cols_), static_cast<void*>(d_snap_)})
         if (p) (void)hipFree(p);
@@ -275,11 +274,13 @@ void DeviceRun::finish_step(Tick const& tk, StepStats const& st, const double* c
     if (opt_.keep_records) rec_.push_back(last_);
 }
 
-// One batch of opt_.batch steps (even) as kernel nodes, one per step: the
-// step with fused statistics, whose last workgroup merges them into
-// ring[s % batch], copies the hot column on cruise steps and advances the
-// device step counter.  `parity` names the physical buffer holding the
-// current field; an even count leaves it there.
+// One batch of opt_.batch steps (even) as kernel nodes, one per step j,
+// plus a flush: node j computes step j with fused block statistics while its
+// extra workgroup merges step j-1's into ring[j-1] and copies that step's hot
+// column on cruise steps (the flush finishes the last step).  Laser ticks and
+// cruise flags are read from the batch copies, refreshed before each replay,
+// so no node depends on a device counter.  `parity` names the physical
+// buffer holding the current field; an even count leaves it there.
 void DeviceRun::capture(int parity) {
     const hipStream_t s = S(stream_);
     const int B = opt_.batch;
@@ -289,11 +290,12 @@ void DeviceRun::capture(int parity) {
     hipGraph_t g = nullptr;
     hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
     for (int i = 0; i < B; ++i) {
-        lpbf_launch_step_ex_plan(cur, nxt, &k_, phys_, opt_.bc, opt_.T_bc, Tm_, d_plan_, d_step_,
-                                 d_cruise_, static_cast<StepStats*>(d_blk_), d_step_ + 1, d_ring_,
-                                 B, d_cols_, s);
+        lpbf_launch_step_ex_batch(cur, nxt, &k_, phys_, opt_.bc, opt_.T_bc, Tm_, d_bticks_, i,
+                                  d_bcruise_, static_cast<StepStats*>(d_blk_), d_ring_, d_cols_, s);
         std::swap(cur, nxt);
     }
+    lpbf_launch_step_ex_batch_flush(cur, nx_, ny_, B, d_bcruise_,
+                                    static_cast<const StepStats*>(d_blk_), d_ring_, d_cols_, s);
     const hipError_t e = hipStreamEndCapture(s, &g);
     if (e != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
@@ -307,15 +309,16 @@ void DeviceRun::capture(int parity) {
 }
 
 bool DeviceRun::graph_batch(int s0, int bn) {
-    if (opt_.precision != 64 || !d_step_ || bn != opt_.batch || s0 % opt_.batch != 0) return false;
+    if (opt_.precision != 64 || !d_bticks_ || bn != opt_.batch || s0 % opt_.batch != 0) return false;
     if (snap_cb_)
         for (int i = 0; i < bn; ++i)
             if (plan_[static_cast<std::size_t>(s0 + i)].dump) return false;
     const int parity = (d_a_ == buf0_) ? 0 : 1;
     if (!gexec_[parity]) capture(parity);
-    *h_step_ = s0;
-    hip_ok(hipMemcpyAsync(d_step_, h_step_, sizeof(int), hipMemcpyHostToDevice, S(stream_)),
-           "step counter");
+    hip_ok(hipMemcpyAsync(d_bticks_, d_plan_ + s0, static_cast<std::size_t>(bn) * sizeof(LaserTick),
+                          hipMemcpyDeviceToDevice, S(stream_)), "batch ticks");
+    hip_ok(hipMemcpyAsync(d_bcruise_, d_cruise_ + s0, static_cast<std::size_t>(bn),
+                          hipMemcpyDeviceToDevice, S(stream_)), "batch cruise flags");
     hip_ok(hipGraphLaunch(static_cast<hipGraphExec_t>(gexec_[parity]), S(stream_)),
            "graph launch");
     ++graph_launches_;

@@ -34,7 +34,7 @@ constexpr int kRowsPerWave = 32;
 constexpr int kAhead = 3;                    // rows in flight ahead of the one computed
 
 enum { BC_MIRROR = LPBF_BC_MIRROR, BC_DIRICHLET = LPBF_BC_DIRICHLET, BC_TILE = 2 };
-enum { RED_NONE = 0, RED_PARTIAL = 1, RED_STATS = 2, RED_RING = 3 };
+enum { RED_NONE = 0, RED_PARTIAL = 1, RED_STATS = 2, RED_RING = 3, RED_BATCH = 4 };
 
 // The rows being updated.  For BC_TILE, in/out point at the first owned row
 // and rows -1 and ny (the halos) exist in both buffers.
@@ -54,7 +54,9 @@ struct Region {
     // plan[*step] instead of k's x_l, y_l, q_scale; with RED_RING the last
     // workgroup to finish (ticket) merges the block records into
     // ring[s % ring_len], copies the hot column on cruise steps, resets the
-    // ticket and advances *step
+    // ticket and advances *step.  With RED_BATCH the step is the launch
+    // argument j (plan[j], cruise[j]) and an extra last workgroup finishes
+    // step j-1 (see k_step_f64).
     const LaserTick* plan;
     int* step;
     int* ticket;
@@ -64,11 +66,41 @@ struct Region {
     double* cols;
 };
 
+// Finishes step j of a batch: merges its nblk block records (in a fixed
+// order, so the sums are reproducible) into ring[j] and, on cruise steps,
+// copies the column of the step's maximum from `field` (the step's output)
+// into cols[j*ny ...].  One 256-thread workgroup.
+__device__ void finish_batch_step(const StepStats* __restrict__ rec, int nblk, int nx, int ny,
+                                  int j, const double* __restrict__ field,
+                                  const unsigned char* __restrict__ cruise,
+                                  StepStats* __restrict__ ring, double* __restrict__ cols) {
+    __shared__ int s_xhot;
+    StepStats* slot = ring + j;
+    block_merge_stats(rec, nblk, nx, ny, slot);
+    if (threadIdx.x == 0) s_xhot = static_cast<int>(slot->arg % nx);
+    __syncthreads();
+    if (cols && cruise && cruise[j]) {
+        double* col = cols + static_cast<long long>(j) * ny;
+        for (int y = threadIdx.x; y < ny; y += kThreads)
+            col[y] = field[static_cast<long long>(y) * nx + s_xhot];
+    }
+}
+
 template <int PHYS, int RED>
 __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                                                        void* red_out, int red_fill) {
+    if (RED == RED_BATCH && blockIdx.x == gridDim.x - 1) {
+        // the extra workgroup finishes the previous step of the batch while
+        // the others compute this one: its records are in the other half of
+        // the record buffer and its output is this launch's input
+        const int j = red_fill, nblk = static_cast<int>(gridDim.x) - 1;
+        if (j > 0)
+            finish_batch_step(static_cast<const StepStats*>(red_out) + ((j - 1) & 1) * nblk, nblk,
+                              g.nx, g.ny, j - 1, g.in, g.cruise, g.ring, g.cols);
+        return;
+    }
     if (g.plan) {
-        const LaserTick t = g.plan[*g.step];
+        const LaserTick t = g.plan[RED == RED_BATCH ? red_fill : *g.step];
         k.x_l = t.x_l;
         k.y_l = t.y_l;
         k.q_scale = t.q_scale;
@@ -191,7 +223,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                         if (yg > pa.ymax) pa.ymax = yg;
                         ++molten;
                     }
-                    if ((RED == RED_STATS || RED == RED_RING) && upd) {
+                    if ((RED == RED_STATS || RED == RED_RING || RED == RED_BATCH) && upd) {
                         s_dT += v - T;
                         s_src += src;
                         s_loss += lossv;
@@ -210,7 +242,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
     const PartialAcc w = wave_reduce(pa);
     double w_dT = 0.0, w_src = 0.0, w_loss = 0.0;
     long long w_mol = 0;
-    if (RED == RED_STATS || RED == RED_RING) {
+    if (RED == RED_STATS || RED == RED_RING || RED == RED_BATCH) {
         w_dT = wave_sum(s_dT);
         w_src = wave_sum(s_src);
         w_loss = wave_sum(s_loss);
@@ -239,7 +271,9 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                 st.sum_src += s_sum[1][i];
                 st.sum_loss += s_sum[2][i];
             }
-            static_cast<StepStats*>(red_out)[blockIdx.x] = st;
+            StepStats* recs = static_cast<StepStats*>(red_out);
+            if (RED == RED_BATCH) recs += (red_fill & 1) * (static_cast<int>(gridDim.x) - 1);
+            recs[blockIdx.x] = st;
         }
     }
     if (RED == RED_RING) {
@@ -332,6 +366,15 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const Partial* __restrict
 }
 
 // Same for StepStats; the FP64 sums use a fixed order (deterministic).
+__global__ __launch_bounds__(kThreads) void k_batch_flush(const StepStats* __restrict__ rec,
+                                                          int nblk, int nx, int ny, int j,
+                                                          const double* __restrict__ field,
+                                                          const unsigned char* __restrict__ cruise,
+                                                          StepStats* __restrict__ ring,
+                                                          double* __restrict__ cols) {
+    finish_batch_step(rec, nblk, nx, ny, j, field, cruise, ring, cols);
+}
+
 __global__ __launch_bounds__(kThreads) void k_stats_finalize(const StepStats* __restrict__ parts,
                                                              int nblocks, int nx, int ny,
                                                              StepStats* __restrict__ out) {
@@ -397,7 +440,8 @@ int grid_of(const Region& g) {
 template <int RED>
 void launch_step(const Region& g, const StepConst& k, int phys, void* red, int fill,
                  hipStream_t s) {
-    const dim3 grid(grid_of(g)), block(kThreads);
+    // the batch form has one more workgroup, which finishes the previous step
+    const dim3 grid(grid_of(g) + (RED == RED_BATCH ? 1 : 0)), block(kThreads);
     if (phys == LPBF_PHYS_EXTENDED)
         hipLaunchKernelGGL((k_step_f64<LPBF_PHYS_EXTENDED, RED>), grid, block, 0, s, g, k, red, fill);
     else
@@ -545,6 +589,41 @@ void lpbf_launch_step_ex_plan(const double* Tin, double* Tout, const StepConst* 
     g.cruise = d_cruise;
     g.cols = cols;
     launch_step<RED_RING>(g, *k, phys, block_stats, 0, stream);
+}
+
+int lpbf_step_ex_batch_nblocks(int nx, int ny) { return lpbf_step_ex_plan_nblocks(nx, ny); }
+
+void lpbf_launch_step_ex_batch(const double* Tin, double* Tout, const StepConst* k, int phys,
+                               int bc, double T_bc, double Tm, const LaserTick* d_ticks, int j,
+                               const unsigned char* d_cruise, StepStats* block_stats,
+                               StepStats* ring, double* cols, hipStream_t stream) {
+    if (!valid_grid(k) || !Tin || !Tout || Tin == Tout || !d_ticks || j < 0 || !block_stats ||
+        !ring || (phys != 0 && phys != 1) ||
+        (bc != LPBF_BC_MIRROR && bc != LPBF_BC_DIRICHLET)) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    Region g = plan_region(Tin, Tout, k->nx, k->ny, bc, T_bc, Tm);
+    g.plan = d_ticks;
+    g.ring = ring;
+    g.ring_len = 0;
+    g.cruise = d_cruise;
+    g.cols = cols;
+    launch_step<RED_BATCH>(g, *k, phys, block_stats, j, stream);
+}
+
+void lpbf_launch_step_ex_batch_flush(const double* T, int nx, int ny, int j,
+                                     const unsigned char* d_cruise, const StepStats* block_stats,
+                                     StepStats* ring, double* cols, hipStream_t stream) {
+    if (!T || !shape_ok(nx, ny) || j < 1 || !block_stats || !ring) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    const int nblk = lpbf_step_ex_batch_nblocks(nx, ny);
+    hipLaunchKernelGGL(k_batch_flush, dim3(1), dim3(kThreads), 0, stream,
+                       block_stats + ((j - 1) & 1) * nblk, nblk, nx, ny, j - 1, T, d_cruise, ring,
+                       cols);
+    check_launch();
 }
 
 void lpbf_launch_stats_finalize(const StepStats* parts, int nblocks, int nx, int ny,

@@ -119,11 +119,12 @@ private:
     double* d_snap_ = nullptr;     // FP32 path: FP64 staging for snapshots
     StepStats* h_ring_ = nullptr;  // pinned
     double* h_cols_ = nullptr;     // pinned
-    // FP64 graph replay: step counter and cruise flags on the device, one
-    // captured batch per physical buffer holding the current field
-    int* d_step_ = nullptr;
-    int* h_step_ = nullptr;        // pinned
+    // FP64 graph replay: laser ticks and cruise flags of the whole run, the
+    // current batch's copies the captured launches read, one captured batch
+    // per physical buffer holding the current field
     unsigned char* d_cruise_ = nullptr;
+    LaserTick* d_bticks_ = nullptr;
+    unsigned char* d_bcruise_ = nullptr;
     void* buf0_ = nullptr;
     void* graph_[2] = {nullptr, nullptr};
     void* gexec_[2] = {nullptr, nullptr};

@@ -106,6 +106,32 @@ void lpbf_launch_step_ex_plan(const double* Tin, double* Tout, const StepConst* 
                               int* d_ticket, StepStats* ring, int ring_len, double* cols,
                               hipStream_t stream);
 
+/* Batch form of the plan-driven step, for a captured sequence of launches
+ * j = 0..n-1 (every node carries its own j, so there is no device counter
+ * and no atomic): the caller copies the batch's laser ticks and cruise flags
+ * to d_ticks[0..n) and d_cruise[0..n) (device) before each replay.  Launch j
+ * steps Tin -> Tout with the laser of d_ticks[j] and writes its block
+ * records into block_stats[(j % 2) * nblk ...] (nblk =
+ * lpbf_step_ex_batch_nblocks(nx, ny); block_stats holds 2 * nblk records);
+ * one extra workgroup meanwhile finishes step j-1: it merges that step's
+ * records into ring[j-1] and, when d_cruise[j-1] != 0, copies the column of
+ * its maximum from Tin (step j-1's output) into cols[(j-1)*ny ...].
+ * lpbf_launch_step_ex_batch_flush(T_n, ..., j = n, ...) then finishes step
+ * n-1 from its output field.  Fields and columns equal those of
+ * lpbf_launch_step_ex + lpbf_launch_stats_finalize + lpbf_launch_extract_col
+ * bit for bit, the ring's max, index, rows and molten count too; its FP64
+ * sums agree to rounding (the block decomposition differs). */
+int  lpbf_step_ex_batch_nblocks(int nx, int ny);
+void lpbf_launch_step_ex_batch(const double* Tin, double* Tout, const StepConst* k,
+                               int phys, int bc, double T_bc, double Tm,
+                               const LaserTick* d_ticks, int j, const unsigned char* d_cruise,
+                               StepStats* block_stats, StepStats* ring, double* cols,
+                               hipStream_t stream);
+void lpbf_launch_step_ex_batch_flush(const double* T, int nx, int ny, int j,
+                                     const unsigned char* d_cruise,
+                                     const StepStats* block_stats, StepStats* ring,
+                                     double* cols, hipStream_t stream);
+
 /* Merge block StepStats in a fixed order (deterministic FP64 sums). */
 void lpbf_launch_stats_finalize(const StepStats* parts, int nblocks, int nx,
                                 int ny, StepStats* out, hipStream_t stream);

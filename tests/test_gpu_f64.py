@@ -261,3 +261,68 @@ def test_invalid_argument_reported_without_launch():
     h.lib().lpbf_launch_step(16, 32, k, 0, None)
     with pytest.raises(h.LpbfError):
         h.check()
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_step_ex_batch_matches_step_ex(ora, bc):
+    """The counter-free batch launches (step j plus an extra workgroup that
+    finishes step j-1, then a flush) give the field and hot columns of
+    lpbf_launch_step_ex + lpbf_launch_stats_finalize + lpbf_launch_extract_col
+    bit for bit, and the same per-step max, index, molten rows and count; the
+    FP64 energy sums agree to rounding (the batch form uses shorter bands, so
+    its block records split the sums differently)."""
+    print(f"\n[case] step_ex_batch bc={bc}", flush=True)
+    nx, ny, n = 240, 120, 6
+    s = _case(ora, nx, ny)
+    k = ora.make_const(s, ora.dt_cfl(s))
+    T = _rand_field(ny, nx, 77, 300.0, 1900.0)
+    plan = np.zeros((n, 3))
+    plan[:, 0] = (0.3 + 0.02 * np.arange(n)) * s.dom.Lx
+    plan[:, 1] = 0.5 * s.dom.Ly
+    plan[:, 2] = 1.0
+    cruise = np.array([1, 0, 1, 1, 0, 1], dtype=np.uint8)
+    st = stream()
+    lib = h.lib()
+    # reference sequence
+    nb = lib.lpbf_step_ex_nblocks(nx, ny)
+    blk = dev_bytes(nb * 56)
+    ring_r = dev_bytes(56 * n)
+    cols_r = dev(np.zeros((n, ny)))
+    a, b = dev(T), dev(np.zeros_like(T))
+    kk = kconst(k)
+    for j in range(n):
+        kk.x_l, kk.y_l, kk.q_scale = plan[j]
+        lib.lpbf_launch_step_ex(a.data_ptr(), b.data_ptr(), kk, 0, bc, 300.0, 1609.0,
+                                blk.data_ptr(), st)
+        a, b = b, a
+        lib.lpbf_launch_stats_finalize(blk.data_ptr(), nb, nx, ny, ring_r.data_ptr() + 56 * j, st)
+        if cruise[j]:
+            lib.lpbf_launch_extract_col(a.data_ptr(), nx, ny, ring_r.data_ptr() + 56 * j,
+                                        cols_r.data_ptr() + 8 * ny * j, st)
+    field_r = a
+    # batch sequence
+    nbb = lib.lpbf_step_ex_batch_nblocks(nx, ny)
+    blk2 = dev_bytes(2 * nbb * 56)
+    ring = dev_bytes(56 * n)
+    cols = dev(np.zeros((n, ny)))
+    ticks = dev(plan)
+    cr = dev(cruise)
+    a, b = dev(T), dev(np.zeros_like(T))
+    for j in range(n):
+        lib.lpbf_launch_step_ex_batch(a.data_ptr(), b.data_ptr(), kconst(k), 0, bc, 300.0, 1609.0,
+                                      ticks.data_ptr(), j, cr.data_ptr(), blk2.data_ptr(),
+                                      ring.data_ptr(), cols.data_ptr(), st)
+        a, b = b, a
+    lib.lpbf_launch_step_ex_batch_flush(a.data_ptr(), nx, ny, n, cr.data_ptr(), blk2.data_ptr(),
+                                        ring.data_ptr(), cols.data_ptr(), st)
+    sync_check()
+    assert np.array_equal(a.cpu().numpy().view(np.uint64), field_r.cpu().numpy().view(np.uint64))
+    for g, r in zip(stats_np(ring, n), stats_np(ring_r, n)):
+        assert (g.tmax, g.arg, g.ymin, g.ymax, g.molten) == (r.tmax, r.arg, r.ymin, r.ymax,
+                                                             r.molten)
+        for f in ("sum_dT", "sum_src", "sum_loss"):
+            assert getattr(g, f) == pytest.approx(getattr(r, f), rel=1e-12, abs=1e-12)
+    got_c, ref_c = cols.cpu().numpy(), cols_r.cpu().numpy()
+    for j in range(n):
+        if cruise[j]:
+            assert np.array_equal(got_c[j], ref_c[j]), j
