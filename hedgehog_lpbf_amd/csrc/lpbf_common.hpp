@@ -121,9 +121,10 @@ __host__ __device__ inline StepStats stats_identity(int nx, int ny) {
 
 // Merge parts[0..n) into *out with one 256-thread workgroup: thread t folds
 // t, t+256, ... in order, then a fixed tree, so FP64 sums are reproducible
-// and the Partial prefix equals the serial merge.
-__device__ inline void block_merge_stats(const StepStats* __restrict__ parts, int n, int nx,
-                                         int ny, StepStats* __restrict__ out) {
+// and the Partial prefix equals the serial merge.  Thread 0 also returns the
+// merged record (the others return an unspecified value).
+__device__ inline StepStats block_merge_stats(const StepStats* __restrict__ parts, int n, int nx,
+                                              int ny, StepStats* __restrict__ out) {
     constexpr int kWaves = 256 / kWave;
     PartialAcc p = PartialAcc::identity(nx, ny);
     double sd = 0.0, ss = 0.0, sl = 0.0;
@@ -157,7 +158,9 @@ __device__ inline void block_merge_stats(const StepStats* __restrict__ parts, in
         }
         st.tmax = b.tmax; st.arg = b.arg; st.ymin = b.ymin; st.ymax = b.ymax;
         *out = st;
+        return st;
     }
+    return StepStats{};
 }
 
 // SplitMix64 output function (Steele, Lea & Flood, OOPSLA 2014) applied to

@@ -75,9 +75,8 @@ __device__ void finish_batch_step(const StepStats* __restrict__ rec, int nblk, i
                                   const unsigned char* __restrict__ cruise,
                                   StepStats* __restrict__ ring, double* __restrict__ cols) {
     __shared__ int s_xhot;
-    StepStats* slot = ring + j;
-    block_merge_stats(rec, nblk, nx, ny, slot);
-    if (threadIdx.x == 0) s_xhot = static_cast<int>(slot->arg % nx);
+    const StepStats st = block_merge_stats(rec, nblk, nx, ny, ring + j);
+    if (threadIdx.x == 0) s_xhot = static_cast<int>(st.arg % nx);
     __syncthreads();
     if (cols && cruise && cruise[j]) {
         double* col = cols + static_cast<long long>(j) * ny;
