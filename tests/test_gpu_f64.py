@@ -326,3 +326,88 @@ def test_step_ex_batch_matches_step_ex(ora, bc):
     for j in range(n):
         if cruise[j]:
             assert np.array_equal(got_c[j], ref_c[j]), j
+
+
+def test_step_ex_plan_ring_matches_step_ex(ora):
+    """The ticket-merged plan-driven step (lpbf_launch_step_ex_plan): six
+    launches advance the device step counter, fill the ring and copy the hot
+    columns on cruise steps; field and columns equal the step_ex path bit for
+    bit, the ring's max, index, rows and molten count exactly, its sums to
+    rounding; the counter ends at six and the ticket back at zero."""
+    print("\n[case] step_ex_plan ring", flush=True)
+    nx, ny, n = 240, 120, 6
+    s = _case(ora, nx, ny)
+    k = ora.make_const(s, ora.dt_cfl(s))
+    T = _rand_field(ny, nx, 78, 300.0, 1900.0)
+    plan = np.zeros((n, 3))
+    plan[:, 0] = (0.3 + 0.02 * np.arange(n)) * s.dom.Lx
+    plan[:, 1] = 0.5 * s.dom.Ly
+    plan[:, 2] = 1.0
+    cruise = np.array([0, 1, 1, 0, 1, 1], dtype=np.uint8)
+    st = stream()
+    lib = h.lib()
+    nb = lib.lpbf_step_ex_nblocks(nx, ny)
+    blk = dev_bytes(nb * 56)
+    ring_r = dev_bytes(56 * n)
+    cols_r = dev(np.zeros((n, ny)))
+    a, b = dev(T), dev(np.zeros_like(T))
+    kk = kconst(k)
+    for j in range(n):
+        kk.x_l, kk.y_l, kk.q_scale = plan[j]
+        lib.lpbf_launch_step_ex(a.data_ptr(), b.data_ptr(), kk, 0, 0, 300.0, 1609.0,
+                                blk.data_ptr(), st)
+        a, b = b, a
+        lib.lpbf_launch_stats_finalize(blk.data_ptr(), nb, nx, ny, ring_r.data_ptr() + 56 * j, st)
+        if cruise[j]:
+            lib.lpbf_launch_extract_col(a.data_ptr(), nx, ny, ring_r.data_ptr() + 56 * j,
+                                        cols_r.data_ptr() + 8 * ny * j, st)
+    field_r = a
+    npb = lib.lpbf_step_ex_plan_nblocks(nx, ny)
+    blk2 = dev_bytes(npb * 56)
+    ring = dev_bytes(56 * n)
+    cols = dev(np.zeros((n, ny)))
+    counters = dev(np.zeros(2, dtype=np.int32))      # step counter, ticket
+    pd, cr = dev(plan), dev(cruise)
+    a, b = dev(T), dev(np.zeros_like(T))
+    for j in range(n):
+        lib.lpbf_launch_step_ex_plan(a.data_ptr(), b.data_ptr(), kconst(k), 0, 0, 300.0, 1609.0,
+                                     pd.data_ptr(), counters.data_ptr(), cr.data_ptr(),
+                                     blk2.data_ptr(), counters.data_ptr() + 4, ring.data_ptr(), n,
+                                     cols.data_ptr(), st)
+        a, b = b, a
+    sync_check()
+    assert counters.cpu().numpy().tolist() == [n, 0]
+    assert np.array_equal(a.cpu().numpy().view(np.uint64), field_r.cpu().numpy().view(np.uint64))
+    for g, r in zip(stats_np(ring, n), stats_np(ring_r, n)):
+        assert (g.tmax, g.arg, g.ymin, g.ymax, g.molten) == (r.tmax, r.arg, r.ymin, r.ymax,
+                                                             r.molten)
+        for f in ("sum_dT", "sum_src", "sum_loss"):
+            assert getattr(g, f) == pytest.approx(getattr(r, f), rel=1e-12, abs=1e-12)
+    got_c, ref_c = cols.cpu().numpy(), cols_r.cpu().numpy()
+    for j in range(n):
+        if cruise[j]:
+            assert np.array_equal(got_c[j], ref_c[j]), j
+
+
+def test_autotuners_pick_valid_tilings(ora):
+    """lpbf_f32_autotune and lpbf_f64_blocked_autotune on a small plate:
+    they return a depth in range and a band height the launchers accept, and
+    report the error state clean."""
+    import ctypes as C
+    print("\n[case] autotuners", flush=True)
+    s = _case(ora, 300, 200)
+    k = kconst(ora.make_const(s, ora.dt_cfl(s)))
+    plan = np.zeros((8, 3))
+    plan[:, 0], plan[:, 1], plan[:, 2] = 0.4 * s.dom.Lx, 0.5 * s.dom.Ly, 1.0
+    pd = dev(plan)
+    lib = h.lib()
+    d, r, ms = C.c_int(), C.c_int(), C.c_double()
+    assert lib.lpbf_f32_autotune(k, 300.0, 0, 300.0, 1609.0, pd.data_ptr(), 8, 4, stream(),
+                                 C.byref(d), C.byref(r), C.byref(ms)) == 0
+    assert 1 <= d.value <= 4 and r.value >= 0 and ms.value > 0
+    assert lib.lpbf_f32_workspace_bytes(300, 200, d.value, r.value) > 0
+    assert lib.lpbf_f64_blocked_autotune(k, 0, 0, 300.0, 1609.0, pd.data_ptr(), 8, 4, stream(),
+                                         C.byref(d), C.byref(r), C.byref(ms)) == 0
+    assert 1 <= d.value <= 4 and r.value >= 0 and ms.value > 0
+    assert lib.lpbf_f64_blocked_workspace_bytes(300, 200, d.value, r.value) > 0
+    sync_check()
