@@ -129,11 +129,22 @@ __device__ inline StepStats block_merge_stats(const StepStats* __restrict__ part
     PartialAcc p = PartialAcc::identity(nx, ny);
     double sd = 0.0, ss = 0.0, sl = 0.0;
     long long m = 0;
-    for (int i = threadIdx.x; i < n; i += 256) {
-        const StepStats q = parts[i];
+    auto fold = [&](const StepStats& q) {
         p.merge(PartialAcc{q.tmax, q.arg, q.ymin, q.ymax});
         m += q.molten; sd += q.sum_dT; ss += q.sum_src; sl += q.sum_loss;
+    };
+    // four records in flight per thread, folded in the same order as one at
+    // a time (the sums do not change)
+    int i = threadIdx.x;
+    for (; i + 3 * 256 < n; i += 4 * 256) {
+        const StepStats q0 = parts[i], q1 = parts[i + 256], q2 = parts[i + 512],
+                        q3 = parts[i + 768];
+        fold(q0);
+        fold(q1);
+        fold(q2);
+        fold(q3);
     }
+    for (; i < n; i += 256) fold(parts[i]);
     __shared__ PartialAcc s_pa[kWaves];
     __shared__ double s_sum[3][kWaves];
     __shared__ long long s_m[kWaves];
