@@ -12,7 +12,9 @@ plate.  `--workload plate8k_f64` runs the same plate on the FP64 parity path
 autotuned);
 `batch256` steps 256 process-parameter instances of uneven size together
 (one bench step = one time step of every instance); `case0` times the
-reference's published case on the device-resident FP64 path.
+reference's published case on the device-resident FP64 path; `ensemble`
+steps a 64-row manifest sweep of that plate together (lpbf::Ensemble) and
+times the row-by-row loop beside it.
 
 Prints one JSON line: value = cell-updates/s over the timed steps, with the
 inputs already resident in HBM; `roofline` for the dominant kernel (step
@@ -259,6 +261,75 @@ def case0_bench(args):
         "roofline": None, "cpu_baseline": None}), flush=True)
 
 
+def ensemble_bench(args):
+    """The reference's manifest sweep shape: `--runs` rows of the case0 plate
+    (240x120, baseline physics, t_end 1.5 ms) with P, V, r0, eta drawn from the
+    reference case table's ranges, stepped together by lpbf::Ensemble (one FP64
+    launch per step for every row).  value = cell-updates of all rows / wall;
+    the same rows one after another on DeviceRun (the row loop) are timed
+    beside it."""
+    import numpy as np
+    import torch
+
+    import hedgehog_lpbf_amd as h
+    rng = np.random.default_rng(3)
+    runs, t_end = [], 1.5e-3
+    for _ in range(args.runs):
+        mat, pr, dom = h.Material(), h.Process(), h.Domain()
+        pr.P, pr.V = rng.uniform(245.0, 325.0), rng.uniform(0.8, 1.2)
+        pr.r0, mat.eta = rng.uniform(24.5e-6, 41e-6), rng.uniform(0.30, 0.45)
+        dt = h.dt_cfl(mat, dom)
+        n = int(t_end / dt)
+        k = h.make_const(mat, pr, dom, dt)
+        plan, x_csv, wrapped = h.single_track_plan(pr.V, dt, dom.Lx, dom.Ly, n)
+        runs.append(dict(k=k, plan=plan, x_csv=x_csv, wrapped=wrapped, Tamb=pr.Tamb))
+    cells = dom.nx * dom.ny * sum(len(r["plan"]) for r in runs)
+    reps = args.steps if args.steps is not None else 10
+
+    def ens_once():
+        with h.Ensemble(runs, Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, batch=64,
+                        keep_records=False) as e:
+            t0 = time.perf_counter()
+            e.advance(10 ** 9)
+            wall = time.perf_counter() - t0
+            return wall, [e.summary(i)[0] for i in range(len(runs))]
+
+    def loop_once():
+        t0 = time.perf_counter()
+        peaks = []
+        for r in runs:
+            with h.DeviceRun(r["k"], Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, Tamb=r["Tamb"],
+                             plan=r["plan"], x_csv=r["x_csv"], wrapped=r["wrapped"], batch=64,
+                             keep_records=False) as d:
+                d.advance(len(r["plan"]))
+                peaks.append(d.summary()[0])
+        return time.perf_counter() - t0, peaks
+
+    for _ in range(args.warmup if args.warmup is not None else 2):
+        ens_once()
+    torch.cuda.synchronize()
+    ens = [ens_once() for _ in range(reps)]
+    wall = sorted(w for w, _ in ens)[len(ens) // 2]
+    loop = [loop_once() for _ in range(max(1, reps // 5))]
+    lwall = sorted(w for w, _ in loop)[len(loop) // 2]
+    assert ens[0][1] == loop[0][1], "ensemble and row loop disagree on peak T"
+    n_steps = max(len(r["plan"]) for r in runs)
+    print(json.dumps({
+        "metric": "cell-updates/s", "value": cells / wall, "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": n_steps, "warmup": 0, "ms_per_step": wall * 1e3 / n_steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"manifest sweep: {len(runs)} rows of the case0 plate (240x120, "
+                               "baseline physics, 1.5 ms, P U[245,325] W, V U[0.8,1.2] m/s, "
+                               "r0 U[24.5,41] um, eta U[0.30,0.45]) stepped together by "
+                               "lpbf::Ensemble, FP64 parity kernels, batch 64",
+                   "median_of": reps, "rows": len(runs),
+                   "row_loop_cell_updates_per_s": cells / lwall,
+                   "row_loop_wall_ms": lwall * 1e3, "ensemble_wall_ms": wall * 1e3,
+                   "speedup_vs_row_loop": lwall / wall},
+        "roofline": None, "cpu_baseline": None}), flush=True)
+
+
 def surrogate_bench(args):
     """Training of the melt-pool surrogate on FD-generated windows (BF16 MFMA
     forward/backward kernels, Adam lr 1e-3, batch 256), its held-out error
@@ -383,10 +454,11 @@ def main():
                          "20 repetitions for case0)")
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="plate8k_f32",
-                    choices=["plate8k_f32", "plate8k_f64", "batch256", "case0", "surrogate",
-                             "copy"],
+                    choices=["plate8k_f32", "plate8k_f64", "batch256", "case0", "ensemble",
+                             "surrogate", "copy"],
                     help="batch256: 256 uneven instances in one hipGraph launch; case0: the "
-                         "reference's published case on the device-resident path; copy: a "
+                         "reference's published case on the device-resident path; ensemble: "
+                         "--runs manifest rows of that plate stepped together; copy: a "
                          "plain device copy of one n x n FP32 plate per step, the reference "
                          "for calibrating HBM byte counters")
     ap.add_argument("--depth", type=int, default=0,
@@ -395,6 +467,7 @@ def main():
     ap.add_argument("--rows", type=int, default=0,
                     help="plate workloads: band height (0 = built-in)")
     ap.add_argument("--n", type=int, default=8192, help="plate side (cells)")
+    ap.add_argument("--runs", type=int, default=64, help="ensemble: manifest rows")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -412,6 +485,8 @@ def main():
         return batch_bench(args)
     if args.workload == "case0":
         return case0_bench(args)
+    if args.workload == "ensemble":
+        return ensemble_bench(args)
     if args.workload == "surrogate":
         return surrogate_bench(args)
     if args.steps is None:

@@ -17,13 +17,13 @@ from ._lib import (LPBF_BC_DIRICHLET, LPBF_BC_MIRROR, LPBF_PHYS_BASELINE,  # noq
                    LPBF_PHYS_EXTENDED, LaserTick, LpbfError, Partial, RunOptions, StepConst,
                    StepRecord, StepStats, check, lib)
 from .batch import F32Batch, InstanceSpec  # noqa: F401
-from .run import DeviceRun  # noqa: F401
+from .run import DeviceRun, Ensemble  # noqa: F401
 
 __all__ = [
     "Material", "Process", "Domain", "make_const", "dt_cfl", "dt_cfl_ext",
     "single_track_plan", "raster_plan", "centred_raster", "apply_case",
     "StepConst", "Partial", "LaserTick", "StepStats", "lib", "check", "LpbfError",
-    "DeviceRun", "RunOptions", "StepRecord", "F32Batch", "InstanceSpec",
+    "DeviceRun", "Ensemble", "RunOptions", "StepRecord", "F32Batch", "InstanceSpec",
     "LPBF_BC_MIRROR", "LPBF_BC_DIRICHLET", "LPBF_PHYS_BASELINE", "LPBF_PHYS_EXTENDED",
 ]
 

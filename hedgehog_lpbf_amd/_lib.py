@@ -112,6 +112,8 @@ SIGNATURES = {
     "lpbf_launch_step_ex_batch": (_v, [_vp, _vp, _KP, _i, _i, _d, _d, _vp, _i, _vp, _vp, _vp, _vp,
                                        _vp]),
     "lpbf_launch_step_ex_batch_flush": (_v, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "lpbf_launch_step_ensemble": (_v, [_vp, _i, _i, _i, _i, _i, _d, _d, _vp, _i, _vp]),
+    "lpbf_launch_step_ensemble_flush": (_v, [_vp, _i, _i, _i, _vp, _i, _vp]),
     "lpbf_f64_blocked_workspace_bytes": (C.c_size_t, [_i, _i, _i, _i]),
     "lpbf_launch_step_f64_blocked": (_v, [_vp, _vp, _KP, _i, _i, _d, _d, _vp, _i, _i, _i, _vp,
                                           _vp, _vp]),
@@ -162,6 +164,16 @@ SIGNATURES = {
     "lpbf_run_summary": (_i, [_vp, _DP, C.POINTER(C.c_int)]),
     "lpbf_run_records": (_i, [_vp, _i, _i, C.POINTER(StepRecord)]),
     "lpbf_run_graph_launches": (_i, [_vp]),
+    # ensemble of runs on one plate (include/lpbf_run.h)
+    "lpbf_ensemble_create": (_i, [_KP, _i, _i, _d, _d, _d, _vp, _vp, _vp, _vp, _vp,
+                                  C.POINTER(RunOptions), C.POINTER(_vp)]),
+    "lpbf_ensemble_destroy": (_v, [_vp]),
+    "lpbf_ensemble_upload": (_i, [_vp, _i, _vp]),
+    "lpbf_ensemble_advance": (_i, [_vp, _i]),
+    "lpbf_ensemble_download": (_i, [_vp, _i, _vp]),
+    "lpbf_ensemble_summary": (_i, [_vp, _i, _DP, C.POINTER(C.c_int)]),
+    "lpbf_ensemble_graph_launches": (_i, [_vp]),
+    "lpbf_ensemble_records": (_i, [_vp, _i, _i, _i, C.POINTER(StepRecord)]),
     # host helpers (include/lpbf_host.h)
     "lpbf_make_const": (_v, [_DP, _DP, _i, _d, _d, _i, _i, _d, _KP]),
     "lpbf_dt_cfl": (_d, [_DP, _d, _d, _i, _i]),

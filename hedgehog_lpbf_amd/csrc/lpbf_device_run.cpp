@@ -19,49 +19,30 @@
 #include <string>
 #include <utility>
 
+#include "lpbf_host_util.hpp"
 #include "lpbf_kernels.h"
 #include "lpbf_run.h"
 
-namespace lpbf_amd {
-void record_error(int err);
-}
-
 namespace lpbf {
+
+std::mutex& setup_mutex() {
+    static std::mutex m;
+    return m;
+}
 
 namespace {
 
-struct HipFailure : std::runtime_error {
-    int code;
-    HipFailure(int c, std::string const& m) : std::runtime_error(m), code(c) {}
-};
-
-void hip_ok(hipError_t e, const char* what) {
-    if (e != hipSuccess)
-        throw HipFailure(static_cast<int>(e),
-                         std::string("DeviceRun: ") + what + ": " + hipGetErrorString(e));
-}
-
-// Launch errors are recorded by the library; surface them per batch.
-void lib_ok(const char* what) {
-    const int e = lpbf_get_last_error();
-    if (e != 0)
-        throw HipFailure(e, std::string("DeviceRun: ") + what + ": " + lpbf_error_string(e));
-}
+void hip_ok(hipError_t e, const char* what) { hip_check(e, "DeviceRun", what); }
+void lib_ok(const char* what) { lib_check("DeviceRun", what); }
 
 hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
-
-// Runs on several host threads (one stream each) may share the device.  A
-// graph capture must not overlap another thread's allocation, free or
-// device-wide call, so capture, set-up and teardown take this lock; the
-// steady state (graph launches, batch copies and syncs) does not.
-std::mutex g_setup;
 
 }  // namespace
 
 DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<Tick> plan,
                      double Tamb, RunOptions opt)
     : dom_(dom), k_(k), phys_(phys), Tm_(Tm), plan_(std::move(plan)), Tamb_(Tamb), opt_(opt),
-      nx_(dom.nx), ny_(dom.ny) {
+      nx_(dom.nx), ny_(dom.ny), met_(dom, Tm) {
     if (k_.nx != nx_ || k_.ny != ny_ || nx_ < 2 || ny_ < 1)
         throw std::invalid_argument("DeviceRun: StepConst grid does not match the domain");
     if (opt_.precision != 64 && opt_.precision != 32)
@@ -80,7 +61,7 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
 
     const std::size_t N = static_cast<std::size_t>(nx_) * ny_;
     const std::size_t B = static_cast<std::size_t>(opt_.batch);
-    std::lock_guard<std::mutex> lock(g_setup);
+    std::lock_guard<std::mutex> lock(setup_mutex());
     try {
         hipStream_t s;
         hip_ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
@@ -153,7 +134,7 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
 DeviceRun::~DeviceRun() { release(); }
 
 void DeviceRun::release() noexcept {
-    std::lock_guard<std::mutex> lock(g_setup);
+    std::lock_guard<std::mutex> lock(setup_mutex());
     release_locked();
 }
 
@@ -244,14 +225,15 @@ int DeviceRun::advance(int n) {
     return ran;
 }
 
-void DeviceRun::finish_step(Tick const& tk, StepStats const& st, const double* col) {
+StepRecord RunMetrics::finish(Tick const& tk, StepStats const& st, const double* col) {
     // MetricsTask::execute (gpu_step_task.hpp:375-404), same expressions.
+    const int ny = dom_.ny;
     const double dy = dom_.dy();
     const double W_um = (st.ymax < 0) ? 0.0 : (st.ymax - st.ymin + 1) * dy * 1e6;
     double W_sub_um = 0.0;
     if (st.tmax > Tm_ && col) {
         int a = -1, b = -1;
-        for (int y = 0; y < ny_; ++y)
+        for (int y = 0; y < ny; ++y)
             if (col[y] > Tm_) {
                 if (a < 0) a = y;
                 b = y;
@@ -259,18 +241,24 @@ void DeviceRun::finish_step(Tick const& tk, StepStats const& st, const double* c
         if (a >= 0) {
             double y_lo = a * dy, y_hi = b * dy;
             if (a > 0) y_lo = ((a - 1) + (Tm_ - col[a - 1]) / (col[a] - col[a - 1])) * dy;
-            if (b < ny_ - 1) y_hi = (b + (Tm_ - col[b]) / (col[b + 1] - col[b])) * dy;
+            if (b < ny - 1) y_hi = (b + (Tm_ - col[b]) / (col[b + 1] - col[b])) * dy;
             W_sub_um = (y_hi - y_lo) * 1e6;
         }
     }
     if (st.tmax > peak_T_) peak_T_ = st.tmax;
     if (W_um > peak_W_) peak_W_ = W_um;
     if (cruise(tk) && W_sub_um > steady_) steady_ = W_sub_um;
-    last_.step = tk.step;
-    last_.has_col = col != nullptr;
-    last_.stats = st;
-    last_.W_um = W_um;
-    last_.W_sub_um = W_sub_um;
+    StepRecord r;
+    r.step = tk.step;
+    r.has_col = col != nullptr;
+    r.stats = st;
+    r.W_um = W_um;
+    r.W_sub_um = W_sub_um;
+    return r;
+}
+
+void DeviceRun::finish_step(Tick const& tk, StepStats const& st, const double* col) {
+    last_ = met_.finish(tk, st, col);
     if (opt_.keep_records) rec_.push_back(last_);
 }
 
@@ -284,7 +272,7 @@ void DeviceRun::finish_step(Tick const& tk, StepStats const& st, const double* c
 void DeviceRun::capture(int parity) {
     const hipStream_t s = S(stream_);
     const int B = opt_.batch;
-    std::lock_guard<std::mutex> lock(g_setup);
+    std::lock_guard<std::mutex> lock(setup_mutex());
     double* cur = static_cast<double*>(parity == 0 ? buf0_ : (buf0_ == d_a_ ? d_b_ : d_a_));
     double* nxt = static_cast<double*>(cur == d_a_ ? d_b_ : d_a_);
     hipGraph_t g = nullptr;
@@ -417,26 +405,7 @@ struct lpbf_run {
     explicit lpbf_run(A&&... a) : run(std::forward<A>(a)...) {}
 };
 
-namespace {
-
-template <class F>
-int guarded(F&& f) {
-    try {
-        f();
-        return 0;
-    } catch (lpbf::HipFailure const& e) {
-        lpbf_amd::record_error(e.code);
-        return e.code;
-    } catch (std::bad_alloc const&) {
-        lpbf_amd::record_error(static_cast<int>(hipErrorOutOfMemory));
-        return static_cast<int>(hipErrorOutOfMemory);
-    } catch (...) {
-        lpbf_amd::record_error(LPBF_ERR_INVALID_ARGUMENT);
-        return LPBF_ERR_INVALID_ARGUMENT;
-    }
-}
-
-}  // namespace
+using lpbf::guarded;
 
 extern "C" {
 

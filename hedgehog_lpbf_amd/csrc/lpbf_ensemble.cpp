@@ -1,0 +1,391 @@
+// SPDX-License-Identifier: 0BSD
+// lpbf::Ensemble and its C ABI (include/lpbf_run.h, lpbf_ensemble_*).
+//
+// Device layout per run m: two FP64 fields, the laser plan and cruise flags
+// of the whole run, 2 x nblk block records; shared by all runs: the ring
+// [m][batch] of merged per-step statistics and the hot columns
+// [batch][m][ny] (step-major, so the cruise steps of a batch are one
+// contiguous copy), the batch start step, and the lpbf_ensemble_run table
+// the launches read.  Batch step j is one lpbf_launch_step_ensemble over
+// every run; the reference counterpart is a loop of run_one_row calls
+// (src/gpu/hh_gpu_main.cpp:170-198, 234-290).
+#include "lpbf/ensemble.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <mutex>
+#include <stdexcept>
+#include <utility>
+
+#include "lpbf_host_util.hpp"
+#include "lpbf_kernels.h"
+#include "lpbf_run.h"
+
+namespace lpbf {
+
+namespace {
+
+void hip_ok(hipError_t e, const char* what) { hip_check(e, "Ensemble", what); }
+void lib_ok(const char* what) { lib_check("Ensemble", what); }
+
+hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+
+}  // namespace
+
+Ensemble::Ensemble(Domain dom, int phys, double Tm, std::vector<EnsembleMember> runs,
+                   EnsembleOptions opt)
+    : dom_(dom), phys_(phys), Tm_(Tm), runs_(std::move(runs)), opt_(opt), nx_(dom.nx),
+      ny_(dom.ny) {
+    const int n = static_cast<int>(runs_.size());
+    if (n < 1 || n > 65535) throw std::invalid_argument("Ensemble: 1..65535 runs");
+    if (nx_ < 2 || ny_ < 1) throw std::invalid_argument("Ensemble: grid too small");
+    if (phys_ != LPBF_PHYS_BASELINE && phys_ != LPBF_PHYS_EXTENDED)
+        throw std::invalid_argument("Ensemble: unknown physics");
+    if (opt_.bc != LPBF_BC_MIRROR && opt_.bc != LPBF_BC_DIRICHLET)
+        throw std::invalid_argument("Ensemble: unknown boundary condition");
+    if (opt_.batch < 1) throw std::invalid_argument("Ensemble: batch must be >= 1");
+    for (EnsembleMember const& r : runs_) {
+        if (r.k.nx != nx_ || r.k.ny != ny_)
+            throw std::invalid_argument("Ensemble: a run's StepConst grid does not match the domain");
+        if (r.plan.size() > static_cast<std::size_t>(1 << 30))
+            throw std::invalid_argument("Ensemble: plan too long");
+        max_steps_ = std::max(max_steps_, static_cast<int>(r.plan.size()));
+    }
+    nblk_ = lpbf_step_ex_batch_nblocks(nx_, ny_);
+    if (nblk_ <= 0) throw std::invalid_argument("Ensemble: grid too large");
+    if (static_cast<long long>(nblk_ + 1) * n > (1LL << 31) - 1)
+        throw std::invalid_argument("Ensemble: too many runs for one launch");
+    const std::size_t N = static_cast<std::size_t>(nx_) * ny_;
+    const std::size_t B = static_cast<std::size_t>(opt_.batch);
+    const std::size_t nn = static_cast<std::size_t>(n);
+    if (static_cast<long long>(n) * ny_ > (1LL << 31) - 1)
+        throw std::invalid_argument("Ensemble: column pitch too large");
+
+    met_.assign(nn, RunMetrics(dom_, Tm_));
+    rec_.resize(nn);
+    if (opt_.keep_records)
+        for (std::size_t m = 0; m < nn; ++m) rec_[m].reserve(runs_[m].plan.size());
+
+    std::lock_guard<std::mutex> lock(setup_mutex());
+    try {
+        auto dmalloc = [&](std::size_t bytes, const char* what) -> void* {
+            void* p = nullptr;
+            hip_ok(hipMalloc(&p, bytes ? bytes : 1), what);
+            dev_.push_back(p);
+            return p;
+        };
+        hipStream_t s;
+        hip_ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
+        stream_ = s;
+        d_ring_ = static_cast<StepStats*>(dmalloc(nn * B * sizeof(StepStats), "ring"));
+        d_cols_ = static_cast<double*>(dmalloc(nn * B * ny_ * sizeof(double), "columns"));
+        d_s0_ = static_cast<int*>(dmalloc(sizeof(int), "batch start"));
+        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_ring_), nn * B * sizeof(StepStats)),
+               "pinned ring");
+        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_cols_), nn * B * ny_ * sizeof(double)),
+               "pinned columns");
+        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_s0_), sizeof(int)), "pinned batch start");
+
+        std::vector<lpbf_ensemble_run> table(nn);
+        std::vector<double> init(N);
+        for (std::size_t m = 0; m < nn; ++m) {
+            EnsembleMember const& r = runs_[m];
+            const std::size_t ns = r.plan.size();
+            lpbf_ensemble_run& e = table[m];
+            e.k = r.k;
+            e.buf[0] = static_cast<double*>(dmalloc(N * sizeof(double), "field"));
+            e.buf[1] = static_cast<double*>(dmalloc(N * sizeof(double), "field"));
+            buf_.push_back(e.buf[0]);
+            buf_.push_back(e.buf[1]);
+            const std::vector<LaserTick> lt = to_laser(r.plan);
+            std::vector<unsigned char> cr(ns);
+            for (std::size_t i = 0; i < ns; ++i) cr[i] = met_[m].cruise(r.plan[i]) ? 1 : 0;
+            LaserTick* d_plan = static_cast<LaserTick*>(dmalloc(ns * sizeof(LaserTick), "plan"));
+            unsigned char* d_cr = static_cast<unsigned char*>(dmalloc(ns, "cruise flags"));
+            if (ns) {
+                hip_ok(hipMemcpy(d_plan, lt.data(), ns * sizeof(LaserTick), hipMemcpyHostToDevice),
+                       "plan upload");
+                hip_ok(hipMemcpy(d_cr, cr.data(), ns, hipMemcpyHostToDevice), "cruise upload");
+            }
+            e.plan = d_plan;
+            e.cruise = d_cr;
+            e.rec = static_cast<StepStats*>(dmalloc(2 * static_cast<std::size_t>(nblk_) *
+                                                        sizeof(StepStats), "block records"));
+            e.ring = d_ring_ + m * B;
+            e.cols = d_cols_ + m * ny_;
+            e.n_steps = static_cast<int>(ns);
+            e.col_pitch = n * ny_;
+            std::fill(init.begin(), init.end(), r.Tamb);
+            hip_ok(hipMemcpy(e.buf[0], init.data(), N * sizeof(double), hipMemcpyHostToDevice),
+                   "initial field");
+        }
+        d_runs_ = dmalloc(nn * sizeof(lpbf_ensemble_run), "run table");
+        hip_ok(hipMemcpy(d_runs_, table.data(), nn * sizeof(lpbf_ensemble_run),
+                         hipMemcpyHostToDevice), "run table upload");
+    } catch (...) {
+        release();
+        throw;
+    }
+}
+
+Ensemble::~Ensemble() {
+    std::lock_guard<std::mutex> lock(setup_mutex());
+    release();
+}
+
+// Teardown cannot report failures; a sticky device error stays visible to
+// the next HIP call of the caller.  Called with the set-up lock held.
+void Ensemble::release() noexcept {
+    if (stream_) (void)hipStreamSynchronize(S(stream_));
+    if (gexec_) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(gexec_));
+    if (graph_) (void)hipGraphDestroy(static_cast<hipGraph_t>(graph_));
+    gexec_ = graph_ = nullptr;
+    for (void* p : dev_) (void)hipFree(p);
+    dev_.clear();
+    buf_.clear();
+    if (h_ring_) (void)hipHostFree(h_ring_);
+    if (h_cols_) (void)hipHostFree(h_cols_);
+    if (h_s0_) (void)hipHostFree(h_s0_);
+    if (stream_) (void)hipStreamDestroy(S(stream_));
+    h_ring_ = nullptr; h_cols_ = nullptr; h_s0_ = nullptr; stream_ = nullptr;
+    d_runs_ = nullptr; d_ring_ = nullptr; d_cols_ = nullptr; d_s0_ = nullptr;
+}
+
+void Ensemble::upload(int m, const double* T) {
+    if (!T || m < 0 || m >= size()) throw std::invalid_argument("Ensemble::upload: bad run or field");
+    const int cur = std::min(done_, n_steps(m)) & 1;
+    const std::size_t N = static_cast<std::size_t>(nx_) * ny_;
+    hip_ok(hipMemcpyAsync(buf_[2 * static_cast<std::size_t>(m) + cur], T, N * sizeof(double),
+                          hipMemcpyHostToDevice, S(stream_)), "upload");
+    hip_ok(hipStreamSynchronize(S(stream_)), "upload sync");
+}
+
+void Ensemble::download(int m, double* T) const {
+    if (!T || m < 0 || m >= size())
+        throw std::invalid_argument("Ensemble::download: bad run or field");
+    const int cur = std::min(done_, n_steps(m)) & 1;
+    const std::size_t N = static_cast<std::size_t>(nx_) * ny_;
+    hip_ok(hipMemcpyAsync(T, buf_[2 * static_cast<std::size_t>(m) + cur], N * sizeof(double),
+                          hipMemcpyDeviceToHost, S(stream_)), "download");
+    hip_ok(hipStreamSynchronize(S(stream_)), "download sync");
+}
+
+int Ensemble::advance(int n) {
+    int ran = 0;
+    while (ran < n && done_ < max_steps_) {
+        const int bn = std::min({opt_.batch, n - ran, max_steps_ - done_});
+        batch(bn);
+        ran += bn;
+    }
+    return ran;
+}
+
+// bn batch steps of every run and the flush; step parity is global, so the
+// same sequence serves every batch start.
+void Ensemble::launches(int bn) {
+    const hipStream_t s = S(stream_);
+    const auto* runs = static_cast<const lpbf_ensemble_run*>(d_runs_);
+    for (int j = 0; j < bn; ++j)
+        lpbf_launch_step_ensemble(runs, size(), nx_, ny_, phys_, opt_.bc, opt_.T_bc, Tm_, d_s0_, j,
+                                  s);
+    lpbf_launch_step_ensemble_flush(runs, size(), nx_, ny_, d_s0_, bn, s);
+}
+
+void Ensemble::capture() {
+    const hipStream_t s = S(stream_);
+    std::lock_guard<std::mutex> lock(setup_mutex());
+    hipGraph_t g = nullptr;
+    hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+    launches(opt_.batch);
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        hip_ok(e, "end capture");
+    }
+    graph_ = g;
+    lib_ok("captured launches");
+    hipGraphExec_t x = nullptr;
+    hip_ok(hipGraphInstantiateWithFlags(&x, g, 0), "instantiate");
+    gexec_ = x;
+}
+
+void Ensemble::batch(int bn) {
+    const hipStream_t s = S(stream_);
+    const int n = size();
+    const int s0 = done_;
+    const std::size_t B = static_cast<std::size_t>(opt_.batch);
+
+    // batch steps that copy a hot column for some run: [c_lo, c_hi]
+    int c_lo = bn, c_hi = -1;
+    for (int m = 0; m < n; ++m) {
+        const int end = std::min(bn, n_steps(m) - s0);
+        for (int i = 0; i < end; ++i)
+            if (met_[static_cast<std::size_t>(m)].cruise(at(m).plan[static_cast<std::size_t>(s0 + i)])) {
+                c_lo = std::min(c_lo, i);
+                c_hi = std::max(c_hi, i);
+            }
+    }
+
+    *h_s0_ = s0;
+    hip_ok(hipMemcpyAsync(d_s0_, h_s0_, sizeof(int), hipMemcpyHostToDevice, s), "batch start");
+    if (opt_.graph && bn == opt_.batch) {
+        if (!gexec_) capture();
+        hip_ok(hipGraphLaunch(static_cast<hipGraphExec_t>(gexec_), s), "graph launch");
+        ++graph_launches_;
+    } else {
+        launches(bn);
+    }
+    lib_ok("step launches");
+    hip_ok(hipMemcpyAsync(h_ring_, d_ring_, static_cast<std::size_t>(n) * B * sizeof(StepStats),
+                          hipMemcpyDeviceToHost, s), "ring copy");
+    const std::size_t step_cols = static_cast<std::size_t>(n) * ny_;
+    if (c_hi >= c_lo)
+        hip_ok(hipMemcpyAsync(h_cols_ + c_lo * step_cols, d_cols_ + c_lo * step_cols,
+                              static_cast<std::size_t>(c_hi - c_lo + 1) * step_cols * sizeof(double),
+                              hipMemcpyDeviceToHost, s), "column copy");
+    hip_ok(hipStreamSynchronize(s), "batch sync");
+    lib_ok("batch");
+
+    for (int m = 0; m < n; ++m) {
+        RunMetrics& met = met_[static_cast<std::size_t>(m)];
+        const int end = std::min(bn, n_steps(m) - s0);
+        for (int i = 0; i < end; ++i) {
+            Tick const& tk = at(m).plan[static_cast<std::size_t>(s0 + i)];
+            const double* col = met.cruise(tk)
+                                    ? h_cols_ + static_cast<std::size_t>(i) * step_cols +
+                                          static_cast<std::size_t>(m) * ny_
+                                    : nullptr;
+            const StepRecord r =
+                met.finish(tk, h_ring_[static_cast<std::size_t>(m) * B + i], col);
+            if (opt_.keep_records) rec_[static_cast<std::size_t>(m)].push_back(r);
+        }
+    }
+    done_ += bn;
+}
+
+}  // namespace lpbf
+
+// ---- C ABI -------------------------------------------------------------------
+
+struct lpbf_ensemble {
+    lpbf::Ensemble ens;
+    template <class... A>
+    explicit lpbf_ensemble(A&&... a) : ens(std::forward<A>(a)...) {}
+};
+
+using lpbf::guarded;
+
+namespace {
+
+int bad_arg() {
+    lpbf_amd::record_error(LPBF_ERR_INVALID_ARGUMENT);
+    return LPBF_ERR_INVALID_ARGUMENT;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lpbf_ensemble_create(const StepConst* k, int n_runs, int phys, double Tm, double Lx,
+                         double Ly, const double* Tamb, const LaserTick* plan,
+                         const double* x_csv, const unsigned char* wrapped, const int* n_steps,
+                         const lpbf_run_options* opt, lpbf_ensemble** out) {
+    if (!k || !out || !n_steps || n_runs < 1) return bad_arg();
+    *out = nullptr;
+    long long total = 0;
+    for (int m = 0; m < n_runs; ++m) {
+        if (n_steps[m] < 0) return bad_arg();
+        total += n_steps[m];
+    }
+    if (total > 0 && !plan) return bad_arg();
+    if (opt && opt->precision != 64) return bad_arg();
+    return guarded([&] {
+        lpbf::Domain dom;
+        dom.nx = k[0].nx;
+        dom.ny = k[0].ny;
+        dom.Lx = Lx;
+        dom.Ly = Ly;
+        std::vector<lpbf::EnsembleMember> runs(static_cast<std::size_t>(n_runs));
+        long long off = 0;
+        for (int m = 0; m < n_runs; ++m) {
+            lpbf::EnsembleMember& r = runs[static_cast<std::size_t>(m)];
+            r.k = k[m];
+            r.Tamb = Tamb ? Tamb[m] : 300.0;
+            r.plan.resize(static_cast<std::size_t>(n_steps[m]));
+            for (int s = 0; s < n_steps[m]; ++s, ++off) {
+                lpbf::Tick& t = r.plan[static_cast<std::size_t>(s)];
+                t.step = s;
+                t.t = s * k[m].dt;
+                t.x_l = plan[off].x_l;
+                t.y_l = plan[off].y_l;
+                t.q_scale = plan[off].q_scale;
+                t.x_laser_csv = x_csv ? x_csv[off] : -1.0;
+                t.wrapped = x_csv ? (wrapped ? wrapped[off] != 0 : false) : true;
+                t.dump = false;
+            }
+        }
+        lpbf::EnsembleOptions o;
+        if (opt) {
+            o.bc = opt->bc;
+            o.T_bc = opt->T_bc;
+            o.batch = opt->batch;
+            o.keep_records = opt->keep_records != 0;
+            o.graph = opt->graph != 0;
+        }
+        *out = new lpbf_ensemble(dom, phys, Tm, std::move(runs), o);
+    });
+}
+
+void lpbf_ensemble_destroy(lpbf_ensemble* e) { delete e; }
+
+int lpbf_ensemble_upload(lpbf_ensemble* e, int m, const double* T) {
+    if (!e || !T || m < 0 || m >= e->ens.size()) return bad_arg();
+    return guarded([&] { e->ens.upload(m, T); });
+}
+
+int lpbf_ensemble_advance(lpbf_ensemble* e, int n_steps) {
+    if (!e || n_steps < 0) return -bad_arg();
+    int ran = 0;
+    const int rc = guarded([&] { ran = e->ens.advance(n_steps); });
+    return rc ? -rc : ran;
+}
+
+int lpbf_ensemble_download(lpbf_ensemble* e, int m, double* T) {
+    if (!e || !T || m < 0 || m >= e->ens.size()) return bad_arg();
+    return guarded([&] { e->ens.download(m, T); });
+}
+
+int lpbf_ensemble_summary(const lpbf_ensemble* e, int m, double* out3, int* steps_done) {
+    if (!e || !out3 || m < 0 || m >= e->ens.size()) return bad_arg();
+    lpbf::RunMetrics const& met = e->ens.metrics(m);
+    out3[0] = met.peak_T();
+    out3[1] = met.peak_W();
+    out3[2] = met.steady_W_sub();
+    if (steps_done) *steps_done = std::min(e->ens.steps_done(), e->ens.n_steps(m));
+    return 0;
+}
+
+int lpbf_ensemble_graph_launches(const lpbf_ensemble* e) {
+    if (!e) return -bad_arg();
+    return e->ens.graph_launches();
+}
+
+int lpbf_ensemble_records(const lpbf_ensemble* e, int m, int first, int n,
+                          lpbf_step_record* out) {
+    if (!e || !out || m < 0 || m >= e->ens.size() || first < 0 || n < 0 ||
+        static_cast<std::size_t>(first) + static_cast<std::size_t>(n) > e->ens.records(m).size())
+        return bad_arg();
+    auto const& r = e->ens.records(m);
+    for (int i = 0; i < n; ++i) {
+        lpbf::StepRecord const& s = r[static_cast<std::size_t>(first + i)];
+        out[i].stats = s.stats;
+        out[i].W_um = s.W_um;
+        out[i].W_sub_um = s.W_sub_um;
+        out[i].step = s.step;
+        out[i].has_col = s.has_col ? 1 : 0;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -34,7 +34,8 @@ constexpr int kRowsPerWave = 32;
 constexpr int kAhead = 3;                    // rows in flight ahead of the one computed
 
 enum { BC_MIRROR = LPBF_BC_MIRROR, BC_DIRICHLET = LPBF_BC_DIRICHLET, BC_TILE = 2 };
-enum { RED_NONE = 0, RED_PARTIAL = 1, RED_STATS = 2, RED_RING = 3, RED_BATCH = 4 };
+enum { RED_NONE = 0, RED_PARTIAL = 1, RED_STATS = 2, RED_RING = 3, RED_BATCH = 4,
+       RED_ENSEMBLE = 5 };
 
 // The rows being updated.  For BC_TILE, in/out point at the first owned row
 // and rows -1 and ny (the halos) exist in both buffers.
@@ -66,20 +67,24 @@ struct Region {
     double* cols;
 };
 
+// One run of an ensemble (include/lpbf_kernels.h, lpbf_ensemble_run).
+using EnsMember = lpbf_ensemble_run;
+
 // Finishes step j of a batch: merges its nblk block records (in a fixed
 // order, so the sums are reproducible) into ring[j] and, on cruise steps,
 // copies the column of the step's maximum from `field` (the step's output)
-// into cols[j*ny ...].  One 256-thread workgroup.
+// into cols[j*col_pitch ...] (ny values).  One 256-thread workgroup.
 __device__ void finish_batch_step(const StepStats* __restrict__ rec, int nblk, int nx, int ny,
                                   int j, const double* __restrict__ field,
                                   const unsigned char* __restrict__ cruise,
-                                  StepStats* __restrict__ ring, double* __restrict__ cols) {
+                                  StepStats* __restrict__ ring, double* __restrict__ cols,
+                                  long long col_pitch) {
     __shared__ int s_xhot;
     const StepStats st = block_merge_stats(rec, nblk, nx, ny, ring + j);
     if (threadIdx.x == 0) s_xhot = static_cast<int>(st.arg % nx);
     __syncthreads();
     if (cols && cruise && cruise[j]) {
-        double* col = cols + static_cast<long long>(j) * ny;
+        double* col = cols + j * col_pitch;
         for (int y = threadIdx.x; y < ny; y += kThreads)
             col[y] = field[static_cast<long long>(y) * nx + s_xhot];
     }
@@ -88,25 +93,56 @@ __device__ void finish_batch_step(const StepStats* __restrict__ rec, int nblk, i
 template <int PHYS, int RED>
 __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                                                        void* red_out, int red_fill) {
+    // bid: this workgroup within its step's grid (ncb compute workgroups,
+    // plus the finishing one in the batch forms)
+    int bid = blockIdx.x;
+    int ncb = static_cast<int>(gridDim.x) - (RED == RED_BATCH ? 1 : 0);
+    StepStats* recs = static_cast<StepStats*>(red_out);
+    int tick = RED == RED_BATCH ? red_fill : 0;
+    if (RED == RED_ENSEMBLE) {
+        // run m of the ensemble; its step s = s0 + j (s0 = *g.step, the
+        // batch start shared by every run) reads buf[s & 1]
+        ncb = (g.ntasks + kThreads / kWave - 1) / (kThreads / kWave);
+        const int m = static_cast<int>(blockIdx.x) / (ncb + 1);
+        bid = static_cast<int>(blockIdx.x) - m * (ncb + 1);
+        const EnsMember& M = static_cast<const EnsMember*>(red_out)[m];
+        const int j = red_fill, s = *g.step + j;
+        if (bid == ncb) {
+            if (j > 0 && s - 1 < M.n_steps)
+                finish_batch_step(M.rec + ((s - 1) & 1) * ncb, ncb, g.nx, g.ny, j - 1,
+                                  M.buf[s & 1], M.cruise + (s - j), M.ring, M.cols,
+                                  M.col_pitch);
+            return;
+        }
+        if (s >= M.n_steps) return;    // this run has ended
+        g.in = M.buf[s & 1];
+        g.out = M.buf[(s & 1) ^ 1];
+        g.plan = M.plan;
+        k = M.k;
+        recs = M.rec + (s & 1) * ncb;
+        tick = s;
+    }
     if (RED == RED_BATCH && blockIdx.x == gridDim.x - 1) {
         // the extra workgroup finishes the previous step of the batch while
         // the others compute this one: its records are in the other half of
         // the record buffer and its output is this launch's input
-        const int j = red_fill, nblk = static_cast<int>(gridDim.x) - 1;
+        const int j = red_fill, nblk = ncb;
         if (j > 0)
-            finish_batch_step(static_cast<const StepStats*>(red_out) + ((j - 1) & 1) * nblk, nblk,
-                              g.nx, g.ny, j - 1, g.in, g.cruise, g.ring, g.cols);
+            finish_batch_step(recs + ((j - 1) & 1) * nblk, nblk, g.nx, g.ny, j - 1, g.in,
+                              g.cruise, g.ring, g.cols, g.ny);
         return;
     }
+    if (RED == RED_BATCH) recs += (red_fill & 1) * ncb;
     if (g.plan) {
-        const LaserTick t = g.plan[RED == RED_BATCH ? red_fill : *g.step];
+        const LaserTick t =
+            g.plan[(RED == RED_BATCH || RED == RED_ENSEMBLE) ? tick : *g.step];
         k.x_l = t.x_l;
         k.y_l = t.y_l;
         k.q_scale = t.q_scale;
     }
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const int task = blockIdx.x * (kThreads / kWave) + wave;
+    const int task = bid * (kThreads / kWave) + wave;
     const int nx = g.nx, ny = g.ny;
 
     PartialAcc pa = PartialAcc::identity(nx, g.ny_glob);
@@ -222,7 +258,8 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                         if (yg > pa.ymax) pa.ymax = yg;
                         ++molten;
                     }
-                    if ((RED == RED_STATS || RED == RED_RING || RED == RED_BATCH) && upd) {
+                    if ((RED == RED_STATS || RED == RED_RING || RED == RED_BATCH ||
+                         RED == RED_ENSEMBLE) && upd) {
                         s_dT += v - T;
                         s_src += src;
                         s_loss += lossv;
@@ -241,7 +278,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
     const PartialAcc w = wave_reduce(pa);
     double w_dT = 0.0, w_src = 0.0, w_loss = 0.0;
     long long w_mol = 0;
-    if (RED == RED_STATS || RED == RED_RING || RED == RED_BATCH) {
+    if (RED == RED_STATS || RED == RED_RING || RED == RED_BATCH || RED == RED_ENSEMBLE) {
         w_dT = wave_sum(s_dT);
         w_src = wave_sum(s_src);
         w_loss = wave_sum(s_loss);
@@ -270,9 +307,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                 st.sum_src += s_sum[1][i];
                 st.sum_loss += s_sum[2][i];
             }
-            StepStats* recs = static_cast<StepStats*>(red_out);
-            if (RED == RED_BATCH) recs += (red_fill & 1) * (static_cast<int>(gridDim.x) - 1);
-            recs[blockIdx.x] = st;
+            recs[bid] = st;
         }
     }
     if (RED == RED_RING) {
@@ -364,6 +399,18 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const Partial* __restrict
     }
 }
 
+// Finishes the last step of an ensemble batch (j = batch length) of every
+// run, one workgroup per run.
+__global__ __launch_bounds__(kThreads) void k_ensemble_flush(const EnsMember* __restrict__ runs,
+                                                             const int* __restrict__ d_s0, int j,
+                                                             int nblk, int nx, int ny) {
+    const EnsMember& M = runs[blockIdx.x];
+    const int s0 = *d_s0, s = s0 + j;
+    if (j > 0 && s - 1 < M.n_steps)
+        finish_batch_step(M.rec + ((s - 1) & 1) * nblk, nblk, nx, ny, j - 1, M.buf[s & 1],
+                          M.cruise + s0, M.ring, M.cols, M.col_pitch);
+}
+
 // Same for StepStats; the FP64 sums use a fixed order (deterministic).
 __global__ __launch_bounds__(kThreads) void k_batch_flush(const StepStats* __restrict__ rec,
                                                           int nblk, int nx, int ny, int j,
@@ -371,7 +418,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_flush(const StepStats* __res
                                                           const unsigned char* __restrict__ cruise,
                                                           StepStats* __restrict__ ring,
                                                           double* __restrict__ cols) {
-    finish_batch_step(rec, nblk, nx, ny, j, field, cruise, ring, cols);
+    finish_batch_step(rec, nblk, nx, ny, j, field, cruise, ring, cols, ny);
 }
 
 __global__ __launch_bounds__(kThreads) void k_stats_finalize(const StepStats* __restrict__ parts,
@@ -438,9 +485,11 @@ int grid_of(const Region& g) {
 
 template <int RED>
 void launch_step(const Region& g, const StepConst& k, int phys, void* red, int fill,
-                 hipStream_t s) {
-    // the batch form has one more workgroup, which finishes the previous step
-    const dim3 grid(grid_of(g) + (RED == RED_BATCH ? 1 : 0)), block(kThreads);
+                 hipStream_t s, int n_runs = 1) {
+    // the batch forms have one more workgroup per step, which finishes the
+    // previous step; the ensemble form repeats the grid for every run
+    const int per = grid_of(g) + ((RED == RED_BATCH || RED == RED_ENSEMBLE) ? 1 : 0);
+    const dim3 grid(per * n_runs), block(kThreads);
     if (phys == LPBF_PHYS_EXTENDED)
         hipLaunchKernelGGL((k_step_f64<LPBF_PHYS_EXTENDED, RED>), grid, block, 0, s, g, k, red, fill);
     else
@@ -622,6 +671,33 @@ void lpbf_launch_step_ex_batch_flush(const double* T, int nx, int ny, int j,
     hipLaunchKernelGGL(k_batch_flush, dim3(1), dim3(kThreads), 0, stream,
                        block_stats + ((j - 1) & 1) * nblk, nblk, nx, ny, j - 1, T, d_cruise, ring,
                        cols);
+    check_launch();
+}
+
+void lpbf_launch_step_ensemble(const lpbf_ensemble_run* d_runs, int n_runs, int nx, int ny,
+                               int phys, int bc, double T_bc, double Tm, const int* d_s0, int j,
+                               hipStream_t stream) {
+    if (!d_runs || n_runs < 1 || n_runs > 65535 || !shape_ok(nx, ny) || !d_s0 || j < 0 ||
+        (phys != 0 && phys != 1) || (bc != LPBF_BC_MIRROR && bc != LPBF_BC_DIRICHLET) ||
+        static_cast<long long>(lpbf_step_ex_batch_nblocks(nx, ny) + 1) * n_runs > (1LL << 31) - 1) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    Region g = plan_region(nullptr, nullptr, nx, ny, bc, T_bc, Tm);
+    g.step = const_cast<int*>(d_s0);
+    StepConst k{};
+    launch_step<RED_ENSEMBLE>(g, k, phys, const_cast<lpbf_ensemble_run*>(d_runs), j, stream,
+                              n_runs);
+}
+
+void lpbf_launch_step_ensemble_flush(const lpbf_ensemble_run* d_runs, int n_runs, int nx, int ny,
+                                     const int* d_s0, int j, hipStream_t stream) {
+    if (!d_runs || n_runs < 1 || n_runs > 65535 || !shape_ok(nx, ny) || !d_s0 || j < 1) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
+    }
+    hipLaunchKernelGGL(k_ensemble_flush, dim3(n_runs), dim3(kThreads), 0, stream, d_runs, d_s0,
+                       j, lpbf_step_ex_batch_nblocks(nx, ny), nx, ny);
     check_launch();
 }
 

@@ -6,7 +6,10 @@
 //                a pool of --threads writer threads, meta.json (:375-407);
 //   --bench      wall time and steps/s, no files (:409-416);
 //   --quiet      one RESULT line (:201-210, 321-331);
-//   --manifest   a CSV of runs, one RESULT line each, in one process (:234-290).
+//   --manifest   a CSV of runs, one RESULT line each, in one process (:234-290);
+//                --jobs=N runs N rows at once (one stream each), --ensemble
+//                steps the rows of each physics together (lpbf::Ensemble, one
+//                launch per step for all of them).
 //
 // The field stays on the GPU for the whole run (lpbf::DeviceRun); the host
 // synchronises once per --batch steps.  --precision=32 selects the FP32
@@ -32,6 +35,7 @@
 #include <vector>
 
 #include "lpbf/device_run.hpp"
+#include "lpbf/ensemble.hpp"
 #include "lpbf/params.hpp"
 #include "lpbf/plan.hpp"
 #include "lpbf/task_graph.hpp"
@@ -58,6 +62,7 @@ struct CLI {
     std::string manifest;
     int precision = 64, depth = 4, rows = 0;
     int jobs = 1;                  // manifest rows run concurrently (one stream each)
+    bool ensemble = false;         // manifest rows stepped together, one launch per step
     std::string bc = "mirror";
     double T_bc = 300.0;
 };
@@ -83,7 +88,9 @@ struct CLI {
            "  --quiet              emit one compact RESULT csv line (no disk output)\n"
            "  --manifest=FILE      loop a runs CSV in-process\n"
            "  --jobs=N             run N manifest rows at once, each on its own stream\n"
-           "                       (RESULT lines still in manifest order)\n";
+           "                       (RESULT lines still in manifest order)\n"
+           "  --ensemble           step all manifest rows of one physics together (FP64),\n"
+           "                       one launch per step; wall_ms is the group's\n";
     std::exit(rc);
 }
 
@@ -123,6 +130,7 @@ CLI parse_cli(int argc, char** argv) {
         else if (key == "--bc") c.bc = v;
         else if (key == "--Tbc") c.T_bc = f;
         else if (key == "--jobs") c.jobs = n < 1 ? 1 : n;
+        else if (a == "--ensemble") c.ensemble = true;
         else if (key == "--check" || key == "--check-rtol" || key == "--check-atol") {
             // the reference binary runs its CPU stepper here (lpbf_cuda.cu:485-514);
             // this binary carries no CPU solver
@@ -205,8 +213,18 @@ struct RunSpec {
 };
 struct RowResult { double steady_W = 0, peak_T = 0, wall_ms = 0; long n_steps = 0; };
 
-// One run without disk output (reference run_one_row, hh_gpu_main.cpp:170-198).
-RowResult run_one_row(RunSpec const& rs, double t_end, CLI const& cli) {
+// The run a manifest row describes (reference run_one_row,
+// hh_gpu_main.cpp:170-185): default plate and material, the row's process
+// values, the CFL step of its physics, a single track to t_end.
+struct RowSetup {
+    Domain dom;
+    StepConst k;
+    int phys;
+    double Tm, Tamb;
+    std::vector<Tick> plan;
+};
+
+RowSetup row_setup(RunSpec const& rs, double t_end) {
     Material mat;
     Process pr;
     Domain dom;
@@ -217,15 +235,62 @@ RowResult run_one_row(RunSpec const& rs, double t_end, CLI const& cli) {
     const int n_steps = static_cast<int>(t_end / dt);
     StepConst k = make_const(mat, pr, dom, dt);
     k.kmult = rs.kmult;
+    return RowSetup{dom, k, phys == Physics::Extended ? 1 : 0, mat.Tm, pr.Tamb,
+                    single_track_ticks(pr.V, dt, dom.Lx, dom.Ly, n_steps, 0)};
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration_cast<std::chrono::milliseconds>(
+               std::chrono::steady_clock::now() - t0).count();
+}
+
+// One run without disk output (reference run_one_row, hh_gpu_main.cpp:170-198).
+RowResult run_one_row(RunSpec const& rs, double t_end, CLI const& cli) {
+    RowSetup su = row_setup(rs, t_end);
+    const long n_steps = static_cast<long>(su.plan.size());
     RunOptions o = run_options(cli);
     o.keep_records = false;
-    DeviceRun run(dom, k, phys == Physics::Extended ? 1 : 0, mat.Tm,
-                  single_track_ticks(pr.V, dt, dom.Lx, dom.Ly, n_steps, 0), pr.Tamb, o);
+    DeviceRun run(su.dom, su.k, su.phys, su.Tm, std::move(su.plan), su.Tamb, o);
     const auto w0 = std::chrono::steady_clock::now();
     run.run();
-    const double wall_ms = std::chrono::duration_cast<std::chrono::milliseconds>(
-                               std::chrono::steady_clock::now() - w0).count();
-    return RowResult{run.steady_W_sub(), run.peak_T(), wall_ms, static_cast<long>(n_steps)};
+    return RowResult{run.steady_W_sub(), run.peak_T(), ms_since(w0), n_steps};
+}
+
+// --ensemble: the rows of each physics as one lpbf::Ensemble, stepped by
+// one launch per step for all of them.  Every value equals the row-by-row
+// loop's; wall_ms is the wall time of the row's group.
+std::vector<RowResult> run_rows_ensemble(std::vector<RunSpec> const& rows, CLI const& cli) {
+    std::vector<RowResult> res(rows.size());
+    for (int phys = 0; phys < 2; ++phys) {
+        std::vector<std::size_t> idx;
+        std::vector<EnsembleMember> members;
+        Domain dom;
+        double Tm = 0.0;
+        for (std::size_t i = 0; i < rows.size(); ++i) {
+            RowSetup su = row_setup(rows[i], cli.t_end);
+            if (su.phys != phys) continue;
+            dom = su.dom;
+            Tm = su.Tm;
+            members.push_back(EnsembleMember{su.k, std::move(su.plan), su.Tamb});
+            idx.push_back(i);
+        }
+        if (idx.empty()) continue;
+        EnsembleOptions o;
+        const RunOptions ro = run_options(cli);
+        o.bc = ro.bc;
+        o.T_bc = ro.T_bc;
+        o.batch = ro.batch;
+        Ensemble ens(dom, phys, Tm, std::move(members), o);
+        const auto w0 = std::chrono::steady_clock::now();
+        ens.run();
+        const double wall_ms = ms_since(w0);
+        for (std::size_t j = 0; j < idx.size(); ++j) {
+            RunMetrics const& m = ens.metrics(static_cast<int>(j));
+            res[idx[j]] = RowResult{m.steady_W_sub(), m.peak_T(), wall_ms,
+                                    static_cast<long>(ens.n_steps(static_cast<int>(j)))};
+        }
+    }
+    return res;
 }
 
 void emit_result_line(RunSpec const& rs, RowResult const& r) {
@@ -287,7 +352,14 @@ int run_manifest(CLI const& cli) {
         rs.dsub_m = std::atof(v[i_dsub].c_str()); rs.kmult = std::atof(v[i_km].c_str());
         rows.push_back(rs);
     }
-    if (cli.jobs <= 1) {
+    if (cli.ensemble) {
+        if (cli.precision != 64) {
+            std::cerr << "--ensemble steps the FP64 kernels (--precision=64)\n";
+            return 1;
+        }
+        const std::vector<RowResult> res = run_rows_ensemble(rows, cli);
+        for (std::size_t i = 0; i < rows.size(); ++i) emit_result_line(rows[i], res[i]);
+    } else if (cli.jobs <= 1) {
         for (RunSpec const& rs : rows) emit_result_line(rs, run_one_row(rs, cli.t_end, cli));
     } else {
         // worker threads take rows in order; each row owns a DeviceRun and its

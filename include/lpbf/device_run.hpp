@@ -15,10 +15,10 @@
 // Precision 64 runs the FP64 parity kernels, one step per launch, so every
 // per-step value equals the reference's.  A full batch that starts at a
 // multiple of `batch` (even) and holds no snapshot is launched as one
-// captured hipGraph: the laser, ring slot and cruise gate of each step come
-// from device memory indexed by a device step counter, so the same graph
-// replays for every such batch (kernel nodes only).  Precision 32 runs the temporally
-// blocked FP32 kernel with up to `depth` steps per launch; a launch never
+// captured hipGraph: the laser tick and cruise gate of each step come from
+// batch copies in device memory, refreshed before each replay, so the same
+// graph replays for every such batch (kernel nodes only).  Precision 32 runs
+// the temporally blocked FP32 kernel with up to `depth` steps per launch; a launch never
 // crosses a dump step or a batch end, and the hot column (hence W_sub) is
 // taken at the last step of each launch only.
 #pragma once
@@ -53,6 +53,30 @@ struct StepRecord {
     double W_sub_um = 0.0;
 };
 
+// Per-step metrics of one run, finalised on the host in step order from the
+// step's merged statistics and, on cruise steps, the column through the
+// hottest cell (MetricsTask::execute, gpu_step_task.hpp:375-404, same
+// expressions): the molten-row width W, the sub-grid width W_sub, running
+// maxima, and the steady W_sub (the maximum over cruise steps).
+class RunMetrics {
+public:
+    RunMetrics(Domain dom, double Tm) : dom_(dom), Tm_(Tm) {}
+    // cruise gate: first pass, laser between 45 % and 85 % of the plate
+    bool cruise(Tick const& t) const noexcept {
+        return !t.wrapped && t.x_laser_csv >= 0.45 * dom_.Lx && t.x_laser_csv <= 0.85 * dom_.Lx;
+    }
+    // `col` is the ny temperatures of the hot column, or null (no column)
+    StepRecord finish(Tick const& t, StepStats const& st, const double* col);
+    double peak_T() const noexcept { return peak_T_; }
+    double peak_W() const noexcept { return peak_W_; }
+    double steady_W_sub() const noexcept { return steady_; }
+
+private:
+    Domain dom_;
+    double Tm_;
+    double peak_T_ = 0.0, peak_W_ = 0.0, steady_ = 0.0;
+};
+
 class DeviceRun {
 public:
     // dom/k/phys/Tm/plan/Tamb have the meaning of the reference GpuStepTask
@@ -79,9 +103,9 @@ public:
 
     int steps_done() const noexcept { return done_; }
     int n_steps() const noexcept { return static_cast<int>(plan_.size()); }
-    double peak_T() const noexcept { return peak_T_; }
-    double peak_W() const noexcept { return peak_W_; }
-    double steady_W_sub() const noexcept { return steady_; }
+    double peak_T() const noexcept { return met_.peak_T(); }
+    double peak_W() const noexcept { return met_.peak_W(); }
+    double steady_W_sub() const noexcept { return met_.steady_W_sub(); }
     std::vector<StepRecord> const& records() const noexcept { return rec_; }
     StepRecord const& last() const noexcept { return last_; }
     void* stream() const noexcept { return stream_; }
@@ -93,9 +117,7 @@ private:
     void capture(int parity);
     void release() noexcept;
     void release_locked() noexcept;
-    bool cruise(Tick const& t) const noexcept {
-        return !t.wrapped && t.x_laser_csv >= 0.45 * dom_.Lx && t.x_laser_csv <= 0.85 * dom_.Lx;
-    }
+    bool cruise(Tick const& t) const noexcept { return met_.cruise(t); }
     void finish_step(Tick const& t, StepStats const& st, const double* col);
 
     Domain dom_;
@@ -132,7 +154,7 @@ private:
     std::vector<StepRecord> rec_;
     StepRecord last_;
     int done_ = 0;
-    double peak_T_ = 0.0, peak_W_ = 0.0, steady_ = 0.0;
+    RunMetrics met_;
 };
 
 }  // namespace lpbf

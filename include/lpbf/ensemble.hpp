@@ -1,0 +1,104 @@
+// SPDX-License-Identifier: 0BSD
+// Many runs of one plate advanced together: the reference's manifest sweep
+// (src/gpu/hh_gpu_main.cpp:234-290, one run_one_row per CSV row, each a
+// GpuStepTask + MetricsTask pair on the same Domain) as one FP64 launch per
+// step for all rows instead of one small launch per row and step.
+//
+// The default plate (240 x 120) gives a step four or five workgroups: one
+// run leaves nearly every CU idle and is bound by launch latency.  Here
+// every run keeps its own fields, constants, laser plan and cruise flags in
+// HBM, and batch step j of the ensemble is one lpbf_launch_step_ensemble
+// over all runs (per run: the step's compute workgroups plus one that
+// finishes the run's previous step into its ring).  A batch is `batch`
+// such launches and a flush, replayed as one captured hipGraph; one copy
+// of all rings (and of the hot columns when a cruise step is in the batch)
+// to pinned memory and one synchronisation end it, and the host finalises
+// each run's metrics in step order with the same RunMetrics as DeviceRun.
+//
+// Every cell update, block merge and hot-column copy is the one DeviceRun's
+// graph batches make, so fields, per-step statistics, W, W_sub and the
+// steady W_sub equal a DeviceRun of the same run bit for bit.  Runs may
+// differ in their constants (P, V, r0, eta, dSub, kmult, dt, h), plan
+// length and initial field; the grid, physics, boundary condition and Tm
+// are shared.  No snapshots (the manifest loop writes none).
+#pragma once
+
+#include <vector>
+
+#include "../lpbf_step_const.h"
+#include "device_run.hpp"
+#include "params.hpp"
+#include "plan.hpp"
+
+namespace lpbf {
+
+struct EnsembleMember {
+    StepConst k;              // grid equal to the ensemble's; k.kmult honoured
+    std::vector<Tick> plan;   // laser and cruise gate per step (dump ignored)
+    double Tamb = 300.0;      // initial uniform field
+};
+
+struct EnsembleOptions {
+    int bc = LPBF_BC_MIRROR;
+    double T_bc = 300.0;      // Dirichlet edge temperature
+    int batch = 64;           // steps per host synchronisation
+    bool keep_records = false;
+    bool graph = true;        // replay full batches as one captured hipGraph
+};
+
+class Ensemble {
+public:
+    // dom/phys/Tm as for DeviceRun.  Throws std::invalid_argument on bad
+    // input and std::runtime_error on a HIP failure.
+    Ensemble(Domain dom, int phys, double Tm, std::vector<EnsembleMember> runs,
+             EnsembleOptions opt = {});
+    ~Ensemble();
+    Ensemble(Ensemble const&) = delete;
+    Ensemble& operator=(Ensemble const&) = delete;
+
+    int size() const noexcept { return static_cast<int>(runs_.size()); }
+    void upload(int m, const double* T);     // run m's field (nx*ny FP64), before stepping
+    // Advances the shared step counter by up to n steps (every run still
+    // running takes them); returns how many.
+    int advance(int n);
+    int run() { return advance(max_steps_ - done_); }
+    void download(int m, double* T) const;   // run m's current field
+
+    int steps_done() const noexcept { return done_; }      // shared step counter
+    int n_steps(int m) const { return static_cast<int>(at(m).plan.size()); }
+    int max_steps() const noexcept { return max_steps_; }
+    RunMetrics const& metrics(int m) const { return met_.at(static_cast<std::size_t>(m)); }
+    std::vector<StepRecord> const& records(int m) const { return rec_.at(static_cast<std::size_t>(m)); }
+    int graph_launches() const noexcept { return graph_launches_; }
+    void* stream() const noexcept { return stream_; }
+
+private:
+    EnsembleMember const& at(int m) const { return runs_.at(static_cast<std::size_t>(m)); }
+    void batch(int bn);
+    void launches(int bn);
+    void capture();
+    void release() noexcept;
+
+    Domain dom_;
+    int phys_;
+    double Tm_;
+    std::vector<EnsembleMember> runs_;
+    EnsembleOptions opt_;
+    int nx_, ny_, nblk_ = 0, max_steps_ = 0, done_ = 0, graph_launches_ = 0;
+    void* stream_ = nullptr;
+    std::vector<void*> dev_;          // every device allocation, freed in release()
+    void* d_runs_ = nullptr;          // lpbf_ensemble_run[n] (device)
+    std::vector<double*> buf_;        // 2 per run
+    int* d_s0_ = nullptr;             // batch start step (device)
+    int* h_s0_ = nullptr;             // pinned
+    StepStats* d_ring_ = nullptr;     // [n][batch]
+    double* d_cols_ = nullptr;        // [batch][n][ny] (step-major)
+    StepStats* h_ring_ = nullptr;     // pinned
+    double* h_cols_ = nullptr;        // pinned
+    void* graph_ = nullptr;
+    void* gexec_ = nullptr;
+    std::vector<RunMetrics> met_;
+    std::vector<std::vector<StepRecord>> rec_;
+};
+
+}  // namespace lpbf

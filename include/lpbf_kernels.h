@@ -132,6 +132,36 @@ void lpbf_launch_step_ex_batch_flush(const double* T, int nx, int ny, int j,
                                      const StepStats* block_stats, StepStats* ring,
                                      double* cols, hipStream_t stream);
 
+/* Ensemble form of the batch step: many runs of one plate (the reference's
+ * manifest sweeps, src/gpu/hh_gpu_main.cpp:234-290: same Domain, rows that
+ * differ in process parameters) advanced by one launch per batch step.  Each
+ * run has its own constants, two field buffers (step s reads buf[s & 1] and
+ * writes the other, so after n steps the field is in buf[n & 1]), whole-run
+ * laser plan and cruise flags, 2 * nblk block records (nblk =
+ * lpbf_step_ex_batch_nblocks(nx, ny)), and a ring and column buffer for one
+ * batch.  Launch j computes step s = *d_s0 + j of every run still running
+ * (*d_s0 = the batch's first step, in device memory, set by the caller
+ * before the batch) and finishes step s - 1 into ring[j-1] and, on cruise
+ * steps, cols[(j-1)*col_pitch ...] (ny values), exactly as lpbf_launch_step_ex_batch; runs
+ * whose n_steps are reached sit out.  The flush (j = batch length) finishes
+ * the batch's last step.  phys, bc, T_bc and Tm are shared. */
+typedef struct lpbf_ensemble_run {
+    StepConst            k;        /* the run's constants; x_l, y_l, q_scale unused */
+    double*              buf[2];   /* field buffers (device) */
+    const LaserTick*     plan;     /* laser per step (device), n_steps ticks */
+    const unsigned char* cruise;   /* cruise flag per step (device), n_steps bytes */
+    StepStats*           rec;      /* 2 * nblk block records (device) */
+    StepStats*           ring;     /* per batch step (device) */
+    double*              cols;     /* per batch step, ny values each (device) */
+    int                  n_steps;
+    int                  col_pitch; /* doubles from one step's column to the next, >= ny */
+} lpbf_ensemble_run;
+void lpbf_launch_step_ensemble(const lpbf_ensemble_run* d_runs, int n_runs, int nx, int ny,
+                               int phys, int bc, double T_bc, double Tm, const int* d_s0,
+                               int j, hipStream_t stream);
+void lpbf_launch_step_ensemble_flush(const lpbf_ensemble_run* d_runs, int n_runs, int nx,
+                                     int ny, const int* d_s0, int j, hipStream_t stream);
+
 /* Merge block StepStats in a fixed order (deterministic FP64 sums). */
 void lpbf_launch_stats_finalize(const StepStats* parts, int nblocks, int nx,
                                 int ny, StepStats* out, hipStream_t stream);
@@ -235,6 +265,9 @@ void   lpbf_launch_T_to_theta(const double* T, int nx, int ny, double T_ref,
 
 #ifdef __cplusplus
 }
+static_assert(sizeof(lpbf_ensemble_run) == 280, "lpbf_ensemble_run layout");
+#else
+_Static_assert(sizeof(lpbf_ensemble_run) == 280, "lpbf_ensemble_run layout");
 #endif
 
 #endif /* LPBF_KERNELS_H */

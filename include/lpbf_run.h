@@ -72,6 +72,40 @@ int  lpbf_run_graph_launches(const lpbf_run* run);
 /* Copies records [first, first+n) (requires keep_records). */
 int  lpbf_run_records(const lpbf_run* run, int first, int n, lpbf_step_record* out);
 
+/* ---- Ensemble: many runs of one plate, one launch per step for all ------
+ *
+ * The reference's manifest sweep (src/gpu/hh_gpu_main.cpp:234-290: one
+ * run_one_row per CSV row, each a GpuStepTask + MetricsTask pair on the same
+ * Domain) as one handle: n_runs FP64 runs that share the grid, physics,
+ * boundary condition and Tm, and differ in StepConst (k[m]: P, V, r0, eta,
+ * dSub, kmult, dt, ...), laser plan, length and initial field.  Every step
+ * of every run still running is one lpbf_launch_step_ensemble per step for
+ * all runs; the fields, per-step records and summaries equal those of
+ * lpbf_run_* runs of the same inputs bit for bit.
+ *
+ * plan / x_csv / wrapped hold the runs' ticks back to back (run m's start
+ * after the n_steps of runs 0..m-1), with the meaning of lpbf_run_create's;
+ * Tamb[m] is run m's initial uniform field (NULL: 300 K).  opt: precision
+ * must be 64; bc, T_bc, batch, keep_records and graph are used.  Snapshots
+ * are not taken.  Calls on one handle must not overlap. */
+typedef struct lpbf_ensemble lpbf_ensemble;
+int  lpbf_ensemble_create(const StepConst* k, int n_runs, int phys, double Tm, double Lx,
+                          double Ly, const double* Tamb, const LaserTick* plan,
+                          const double* x_csv, const unsigned char* wrapped,
+                          const int* n_steps, const lpbf_run_options* opt,
+                          lpbf_ensemble** out);
+void lpbf_ensemble_destroy(lpbf_ensemble* e);
+int  lpbf_ensemble_upload(lpbf_ensemble* e, int m, const double* T);   /* before or between advances */
+/* Advances the shared step counter by up to n_steps (runs past their own
+ * length sit out); returns the steps advanced or a negative code. */
+int  lpbf_ensemble_advance(lpbf_ensemble* e, int n_steps);
+int  lpbf_ensemble_download(lpbf_ensemble* e, int m, double* T);
+/* Run m: out[0..2] = peak T, peak W, steady W_sub; *steps_done of run m. */
+int  lpbf_ensemble_summary(const lpbf_ensemble* e, int m, double* out3, int* steps_done);
+int  lpbf_ensemble_graph_launches(const lpbf_ensemble* e);
+int  lpbf_ensemble_records(const lpbf_ensemble* e, int m, int first, int n,
+                           lpbf_step_record* out);
+
 #ifdef __cplusplus
 }
 static_assert(sizeof(lpbf_step_record) == 80, "lpbf_step_record layout");

@@ -67,6 +67,33 @@ def test_run_create_rejects_bad_options_before_touching_a_device():
     assert h.lib().lpbf_get_last_error() == 0
 
 
+def test_ensemble_create_rejects_bad_runs_before_touching_a_device():
+    def member(nx=64, ny=32, n=4):
+        k = h.StepConst()
+        k.nx, k.ny, k.dt = nx, ny, 1e-6
+        return dict(k=k, plan=np.zeros((n, 3)))
+    for runs, kw in (([], {}),                                  # no runs
+                     ([member()], dict(phys=3)),                 # unknown physics
+                     ([member()], dict(bc=5)),                   # unknown boundary
+                     ([member()], dict(batch=0)),
+                     ([member(), member(nx=48)], {})):           # grids differ
+        with pytest.raises((h.LpbfError, ValueError)):
+            h.Ensemble(runs, Tm=1609.0, Lx=1e-3, Ly=1e-3, **kw)
+    h.lib().lpbf_get_last_error()
+    # precision other than 64 is refused by the C entry point itself
+    lib, k = h.lib(), (h.StepConst * 1)(member()["k"])
+    opt = h.RunOptions(32, 0, 300.0, 64, 4, 0, -1.0, 0, 1, 0)
+    n = (C.c_int * 1)(4)
+    plan = np.zeros((4, 3))
+    out = C.c_void_p()
+    assert lib.lpbf_ensemble_create(k, 1, 0, 1609.0, 1e-3, 1e-3, None, plan.ctypes.data, None,
+                                    None, n, C.byref(opt), C.byref(out)) == 1000
+    assert not out.value
+    assert lib.lpbf_get_last_error() == 1000
+    assert lib.lpbf_ensemble_advance(None, 1) == -1000
+    lib.lpbf_get_last_error()
+
+
 def test_batch_create_rejects_bad_instances_before_touching_a_device():
     lib = h.lib()
     arr = (h._lib.F32Instance * 2)()
