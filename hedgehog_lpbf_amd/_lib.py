@@ -112,8 +112,10 @@ SIGNATURES = {
     "lpbf_launch_step_ex_batch": (_v, [_vp, _vp, _KP, _i, _i, _d, _d, _vp, _i, _vp, _vp, _vp, _vp,
                                        _vp]),
     "lpbf_launch_step_ex_batch_flush": (_v, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
-    "lpbf_launch_step_ensemble": (_v, [_vp, _i, _i, _i, _i, _i, _d, _d, _vp, _i, _vp]),
-    "lpbf_launch_step_ensemble_flush": (_v, [_vp, _i, _i, _i, _vp, _i, _vp]),
+    "lpbf_step_ensemble_nblocks": (_i, [_i, _i, _i]),
+    "lpbf_step_ensemble_rows": (_i, [_i, _i, _i]),
+    "lpbf_launch_step_ensemble": (_v, [_vp, _i, _i, _i, _i, _i, _i, _d, _d, _vp, _i, _vp]),
+    "lpbf_launch_step_ensemble_flush": (_v, [_vp, _i, _i, _i, _i, _vp, _i, _vp]),
     "lpbf_f64_blocked_workspace_bytes": (C.c_size_t, [_i, _i, _i, _i]),
     "lpbf_launch_step_f64_blocked": (_v, [_vp, _vp, _KP, _i, _i, _d, _d, _vp, _i, _i, _i, _vp,
                                           _vp, _vp]),

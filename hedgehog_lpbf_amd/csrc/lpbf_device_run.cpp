@@ -4,9 +4,10 @@
 // Per batch, on the run's stream and without host synchronisation: step
 // launches with fused statistics, the per-step merge into the device ring,
 // the hot column on cruise steps, the field copy on dump steps; then the
-// rings come back with one copy into pinned memory and one synchronisation,
-// and the host finalises the metrics in step order (reference
-// include/gpu_step_task.hpp:135-239 and MetricsTask::execute, :375-404).
+// rings come back with one copy into pinned memory and an event.  The host
+// queues the next batch, waits for the event and finalises the metrics in
+// step order (reference include/gpu_step_task.hpp:135-239 and
+// MetricsTask::execute, :375-404).
 #include "lpbf/device_run.hpp"
 
 #include <hip/hip_runtime_api.h>
@@ -118,10 +119,16 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
         }
         hip_ok(hipMalloc(reinterpret_cast<void**>(&d_ring_), B * sizeof(StepStats)), "ring");
         hip_ok(hipMalloc(reinterpret_cast<void**>(&d_cols_), B * ny_ * sizeof(double)), "columns");
-        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_ring_), B * sizeof(StepStats)),
+        // two halves: batch b+1 is queued before the host reads batch b's
+        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_ring_), 2 * B * sizeof(StepStats)),
                "pinned ring");
-        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_cols_), B * ny_ * sizeof(double)),
+        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_cols_), 2 * B * ny_ * sizeof(double)),
                "pinned columns");
+        for (int q = 0; q < 2; ++q) {
+            hipEvent_t ev;
+            hip_ok(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+            done_ev_[q] = ev;
+        }
         if (opt_.keep_records) rec_.reserve(plan_.size());
         std::vector<double> init(N, Tamb_);
         upload(init.data());
@@ -158,6 +165,10 @@ void DeviceRun::release_locked() noexcept {
         if (p) (void)hipFree(p);
     if (h_ring_) (void)hipHostFree(h_ring_);
     if (h_cols_) (void)hipHostFree(h_cols_);
+    for (void*& ev : done_ev_) {
+        if (ev) (void)hipEventDestroy(static_cast<hipEvent_t>(ev));
+        ev = nullptr;
+    }
     if (stream_) (void)hipStreamDestroy(S(stream_));
     d_a_ = d_b_ = d_blk_ = nullptr;
     d_plan_ = nullptr; d_ring_ = nullptr; d_cols_ = nullptr; d_snap_ = nullptr;
@@ -213,16 +224,32 @@ void DeviceRun::download(double* T) const {
     lib_ok("download");
 }
 
+// Batches end at multiples of `batch`, so full ones can replay the graph.
+// They are pipelined: batch b+1 is queued (launches, ring and column copies
+// into the other pinned half, an event) before the host waits for batch b
+// and finalises its metrics, so the GPU does not wait for the host.  The
+// device ring and columns are reused in stream order.
 int DeviceRun::advance(int n) {
-    int ran = 0;
-    while (ran < n && done_ < static_cast<int>(plan_.size())) {
-        // batches end at multiples of `batch`, so full ones can replay the graph
-        const int to_edge = opt_.batch - done_ % opt_.batch;
-        const int bn = std::min({to_edge, n - ran, static_cast<int>(plan_.size()) - done_});
-        batch(bn);
-        ran += bn;
+    const int start = done_;
+    const int target = static_cast<int>(std::min<long long>(
+        static_cast<long long>(plan_.size()), static_cast<long long>(done_) + std::max(n, 0)));
+    int queued = done_, slot = 0;
+    Pending cur, nxt;
+    bool have = false;
+    while (have || queued < target) {
+        bool have_n = false;
+        if (queued < target) {
+            const int bn = std::min(opt_.batch - queued % opt_.batch, target - queued);
+            nxt = enqueue(queued, bn, slot);
+            queued += bn;
+            slot ^= 1;
+            have_n = true;
+        }
+        if (have) complete(cur);
+        cur = std::move(nxt);
+        have = have_n;
     }
-    return ran;
+    return done_ - start;
 }
 
 StepRecord RunMetrics::finish(Tick const& tk, StepStats const& st, const double* col) {
@@ -313,13 +340,16 @@ bool DeviceRun::graph_batch(int s0, int bn) {
     return true;
 }
 
-void DeviceRun::batch(int bn) {
+DeviceRun::Pending DeviceRun::enqueue(int s0, int bn, int slot) {
     const hipStream_t s = S(stream_);
     const std::size_t N = static_cast<std::size_t>(nx_) * ny_;
-    struct Pend { int step; std::vector<double> field; };
-    std::vector<Pend> pend;
-    std::vector<char> has_col(static_cast<std::size_t>(bn), 0);
-    const int s0 = done_;
+    Pending p;
+    p.s0 = s0;
+    p.bn = bn;
+    p.slot = slot;
+    std::vector<char>& has_col = p.has_col;
+    has_col.assign(static_cast<std::size_t>(bn), 0);
+    std::vector<Snap>& pend = p.snaps;
 
     if (graph_batch(s0, bn)) {
         for (int i = 0; i < bn; ++i)
@@ -379,20 +409,29 @@ void DeviceRun::batch(int bn) {
         }
     }
     lib_ok("step launches");
-    hip_ok(hipMemcpyAsync(h_ring_, d_ring_, static_cast<std::size_t>(bn) * sizeof(StepStats),
+    const std::size_t B = static_cast<std::size_t>(opt_.batch);
+    hip_ok(hipMemcpyAsync(h_ring_ + slot * B, d_ring_, static_cast<std::size_t>(bn) * sizeof(StepStats),
                           hipMemcpyDeviceToHost, s), "ring copy");
     if (std::find(has_col.begin(), has_col.end(), 1) != has_col.end())
-        hip_ok(hipMemcpyAsync(h_cols_, d_cols_, static_cast<std::size_t>(bn) * ny_ * sizeof(double),
+        hip_ok(hipMemcpyAsync(h_cols_ + slot * B * ny_, d_cols_,
+                              static_cast<std::size_t>(bn) * ny_ * sizeof(double),
                               hipMemcpyDeviceToHost, s), "column copy");
-    hip_ok(hipStreamSynchronize(s), "batch sync");
-    lib_ok("batch");
+    hip_ok(hipEventRecord(static_cast<hipEvent_t>(done_ev_[slot]), s), "batch event");
+    return p;
+}
 
-    for (int i = 0; i < bn; ++i)
-        finish_step(plan_[static_cast<std::size_t>(s0 + i)], h_ring_[i],
-                    has_col[static_cast<std::size_t>(i)]
-                        ? h_cols_ + static_cast<std::size_t>(i) * ny_ : nullptr);
-    done_ += bn;
-    for (auto& p : pend) snap_cb_(p.step, std::move(p.field));
+void DeviceRun::complete(Pending& p) {
+    hip_ok(hipEventSynchronize(static_cast<hipEvent_t>(done_ev_[p.slot])), "batch sync");
+    lib_ok("batch");
+    const std::size_t B = static_cast<std::size_t>(opt_.batch);
+    const StepStats* hr = h_ring_ + p.slot * B;
+    const double* hc = h_cols_ + p.slot * B * ny_;
+    for (int i = 0; i < p.bn; ++i)
+        finish_step(plan_[static_cast<std::size_t>(p.s0 + i)], hr[i],
+                    p.has_col[static_cast<std::size_t>(i)]
+                        ? hc + static_cast<std::size_t>(i) * ny_ : nullptr);
+    done_ += p.bn;
+    for (auto& q : p.snaps) snap_cb_(q.step, std::move(q.field));
 }
 
 }  // namespace lpbf

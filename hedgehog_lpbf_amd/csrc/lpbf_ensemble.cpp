@@ -52,7 +52,10 @@ Ensemble::Ensemble(Domain dom, int phys, double Tm, std::vector<EnsembleMember> 
             throw std::invalid_argument("Ensemble: plan too long");
         max_steps_ = std::max(max_steps_, static_cast<int>(r.plan.size()));
     }
-    nblk_ = lpbf_step_ex_batch_nblocks(nx_, ny_);
+    if (opt_.rows < 0 || opt_.rows > 1024)
+        throw std::invalid_argument("Ensemble: rows must be 0..1024");
+    if (opt_.rows == 0) opt_.rows = lpbf_step_ensemble_rows(nx_, ny_, n);
+    nblk_ = lpbf_step_ensemble_nblocks(nx_, ny_, opt_.rows);
     if (nblk_ <= 0) throw std::invalid_argument("Ensemble: grid too large");
     if (static_cast<long long>(nblk_ + 1) * n > (1LL << 31) - 1)
         throw std::invalid_argument("Ensemble: too many runs for one launch");
@@ -81,11 +84,19 @@ Ensemble::Ensemble(Domain dom, int phys, double Tm, std::vector<EnsembleMember> 
         d_ring_ = static_cast<StepStats*>(dmalloc(nn * B * sizeof(StepStats), "ring"));
         d_cols_ = static_cast<double*>(dmalloc(nn * B * ny_ * sizeof(double), "columns"));
         d_s0_ = static_cast<int*>(dmalloc(sizeof(int), "batch start"));
-        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_ring_), nn * B * sizeof(StepStats)),
+        // two of each host-side buffer: batch b+1 is queued before the host
+        // reads batch b's
+        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_ring_), 2 * nn * B * sizeof(StepStats)),
                "pinned ring");
-        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_cols_), nn * B * ny_ * sizeof(double)),
-               "pinned columns");
-        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_s0_), sizeof(int)), "pinned batch start");
+        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_cols_),
+                             2 * nn * B * ny_ * sizeof(double)), "pinned columns");
+        hip_ok(hipHostMalloc(reinterpret_cast<void**>(&h_s0_), 2 * sizeof(int)),
+               "pinned batch start");
+        for (int q = 0; q < 2; ++q) {
+            hipEvent_t ev;
+            hip_ok(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+            done_ev_[q] = ev;
+        }
 
         std::vector<lpbf_ensemble_run> table(nn);
         std::vector<double> init(N);
@@ -141,6 +152,10 @@ void Ensemble::release() noexcept {
     if (gexec_) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(gexec_));
     if (graph_) (void)hipGraphDestroy(static_cast<hipGraph_t>(graph_));
     gexec_ = graph_ = nullptr;
+    for (void*& ev : done_ev_) {
+        if (ev) (void)hipEventDestroy(static_cast<hipEvent_t>(ev));
+        ev = nullptr;
+    }
     for (void* p : dev_) (void)hipFree(p);
     dev_.clear();
     buf_.clear();
@@ -171,14 +186,30 @@ void Ensemble::download(int m, double* T) const {
     hip_ok(hipStreamSynchronize(S(stream_)), "download sync");
 }
 
+// Batches are pipelined: batch b+1 is queued (launches, ring and column
+// copies into the other pinned half, an event) before the host waits for
+// batch b and finalises its metrics, so the GPU does not wait for the host.
+// The device ring and columns are reused in stream order.
 int Ensemble::advance(int n) {
-    int ran = 0;
-    while (ran < n && done_ < max_steps_) {
-        const int bn = std::min({opt_.batch, n - ran, max_steps_ - done_});
-        batch(bn);
-        ran += bn;
+    const int start = done_;
+    const int target = static_cast<int>(
+        std::min<long long>(max_steps_, static_cast<long long>(done_) + std::max(n, 0)));
+    int queued = done_, slot = 0;
+    Pending cur{}, nxt{};
+    bool have = false;
+    while (have || queued < target) {
+        bool have_n = false;
+        if (queued < target) {
+            nxt = enqueue(queued, std::min(opt_.batch, target - queued), slot);
+            queued += nxt.bn;
+            slot ^= 1;
+            have_n = true;
+        }
+        if (have) complete(cur);
+        cur = nxt;
+        have = have_n;
     }
-    return ran;
+    return done_ - start;
 }
 
 // bn batch steps of every run and the flush; step parity is global, so the
@@ -187,9 +218,9 @@ void Ensemble::launches(int bn) {
     const hipStream_t s = S(stream_);
     const auto* runs = static_cast<const lpbf_ensemble_run*>(d_runs_);
     for (int j = 0; j < bn; ++j)
-        lpbf_launch_step_ensemble(runs, size(), nx_, ny_, phys_, opt_.bc, opt_.T_bc, Tm_, d_s0_, j,
-                                  s);
-    lpbf_launch_step_ensemble_flush(runs, size(), nx_, ny_, d_s0_, bn, s);
+        lpbf_launch_step_ensemble(runs, size(), nx_, ny_, opt_.rows, phys_, opt_.bc, opt_.T_bc,
+                                  Tm_, d_s0_, j, s);
+    lpbf_launch_step_ensemble_flush(runs, size(), nx_, ny_, opt_.rows, d_s0_, bn, s);
 }
 
 void Ensemble::capture() {
@@ -210,25 +241,25 @@ void Ensemble::capture() {
     gexec_ = x;
 }
 
-void Ensemble::batch(int bn) {
+Ensemble::Pending Ensemble::enqueue(int s0, int bn, int slot) {
     const hipStream_t s = S(stream_);
     const int n = size();
-    const int s0 = done_;
     const std::size_t B = static_cast<std::size_t>(opt_.batch);
+    Pending p{s0, bn, slot, bn, -1};
 
     // batch steps that copy a hot column for some run: [c_lo, c_hi]
-    int c_lo = bn, c_hi = -1;
     for (int m = 0; m < n; ++m) {
         const int end = std::min(bn, n_steps(m) - s0);
         for (int i = 0; i < end; ++i)
             if (met_[static_cast<std::size_t>(m)].cruise(at(m).plan[static_cast<std::size_t>(s0 + i)])) {
-                c_lo = std::min(c_lo, i);
-                c_hi = std::max(c_hi, i);
+                p.c_lo = std::min(p.c_lo, i);
+                p.c_hi = std::max(p.c_hi, i);
             }
     }
 
-    *h_s0_ = s0;
-    hip_ok(hipMemcpyAsync(d_s0_, h_s0_, sizeof(int), hipMemcpyHostToDevice, s), "batch start");
+    h_s0_[slot] = s0;
+    hip_ok(hipMemcpyAsync(d_s0_, h_s0_ + slot, sizeof(int), hipMemcpyHostToDevice, s),
+           "batch start");
     if (opt_.graph && bn == opt_.batch) {
         if (!gexec_) capture();
         hip_ok(hipGraphLaunch(static_cast<hipGraphExec_t>(gexec_), s), "graph launch");
@@ -237,31 +268,42 @@ void Ensemble::batch(int bn) {
         launches(bn);
     }
     lib_ok("step launches");
-    hip_ok(hipMemcpyAsync(h_ring_, d_ring_, static_cast<std::size_t>(n) * B * sizeof(StepStats),
+    const std::size_t ring_n = static_cast<std::size_t>(n) * B;
+    hip_ok(hipMemcpyAsync(h_ring_ + slot * ring_n, d_ring_, ring_n * sizeof(StepStats),
                           hipMemcpyDeviceToHost, s), "ring copy");
     const std::size_t step_cols = static_cast<std::size_t>(n) * ny_;
-    if (c_hi >= c_lo)
-        hip_ok(hipMemcpyAsync(h_cols_ + c_lo * step_cols, d_cols_ + c_lo * step_cols,
-                              static_cast<std::size_t>(c_hi - c_lo + 1) * step_cols * sizeof(double),
+    double* hc = h_cols_ + slot * B * step_cols;
+    if (p.c_hi >= p.c_lo)
+        hip_ok(hipMemcpyAsync(hc + p.c_lo * step_cols, d_cols_ + p.c_lo * step_cols,
+                              static_cast<std::size_t>(p.c_hi - p.c_lo + 1) * step_cols *
+                                  sizeof(double),
                               hipMemcpyDeviceToHost, s), "column copy");
-    hip_ok(hipStreamSynchronize(s), "batch sync");
-    lib_ok("batch");
+    hip_ok(hipEventRecord(static_cast<hipEvent_t>(done_ev_[slot]), s), "batch event");
+    return p;
+}
 
+void Ensemble::complete(Pending const& p) {
+    hip_ok(hipEventSynchronize(static_cast<hipEvent_t>(done_ev_[p.slot])), "batch sync");
+    lib_ok("batch");
+    const int n = size();
+    const std::size_t B = static_cast<std::size_t>(opt_.batch);
+    const StepStats* hr = h_ring_ + p.slot * static_cast<std::size_t>(n) * B;
+    const std::size_t step_cols = static_cast<std::size_t>(n) * ny_;
+    const double* hc = h_cols_ + p.slot * B * step_cols;
     for (int m = 0; m < n; ++m) {
         RunMetrics& met = met_[static_cast<std::size_t>(m)];
-        const int end = std::min(bn, n_steps(m) - s0);
+        const int end = std::min(p.bn, n_steps(m) - p.s0);
         for (int i = 0; i < end; ++i) {
-            Tick const& tk = at(m).plan[static_cast<std::size_t>(s0 + i)];
+            Tick const& tk = at(m).plan[static_cast<std::size_t>(p.s0 + i)];
             const double* col = met.cruise(tk)
-                                    ? h_cols_ + static_cast<std::size_t>(i) * step_cols +
+                                    ? hc + static_cast<std::size_t>(i) * step_cols +
                                           static_cast<std::size_t>(m) * ny_
                                     : nullptr;
-            const StepRecord r =
-                met.finish(tk, h_ring_[static_cast<std::size_t>(m) * B + i], col);
+            const StepRecord r = met.finish(tk, hr[static_cast<std::size_t>(m) * B + i], col);
             if (opt_.keep_records) rec_[static_cast<std::size_t>(m)].push_back(r);
         }
     }
-    done_ += bn;
+    done_ += p.bn;
 }
 
 }  // namespace lpbf
@@ -330,6 +372,7 @@ int lpbf_ensemble_create(const StepConst* k, int n_runs, int phys, double Tm, do
             o.bc = opt->bc;
             o.T_bc = opt->T_bc;
             o.batch = opt->batch;
+            o.rows = opt->rows;
             o.keep_records = opt->keep_records != 0;
             o.graph = opt->graph != 0;
         }

@@ -32,6 +32,7 @@ constexpr int kBlockCols = 128;              // columns per wavefront
 constexpr int kThreads = 256;                // 4 wavefronts
 constexpr int kRowsPerWave = 32;
 constexpr int kAhead = 3;                    // rows in flight ahead of the one computed
+constexpr int kMaxEnsembleRows = 1024;       // band height limit of the ensemble launch
 
 enum { BC_MIRROR = LPBF_BC_MIRROR, BC_DIRICHLET = LPBF_BC_DIRICHLET, BC_TILE = 2 };
 enum { RED_NONE = 0, RED_PARTIAL = 1, RED_STATS = 2, RED_RING = 3, RED_BATCH = 4,
@@ -518,6 +519,16 @@ Region plan_region(const double* in, double* out, int nx, int ny, int bc, double
     return g;
 }
 
+// The ensemble launch: rows = 0 takes the single-run band height.
+Region ensemble_region(int nx, int ny, int rows, int bc, double T_bc, double Tm) {
+    Region g = plan_region(nullptr, nullptr, nx, ny, bc, T_bc, Tm);
+    if (rows > 0) {
+        g.R = rows;
+        g.ntasks = g.nwx * ((ny + rows - 1) / rows);
+    }
+    return g;
+}
+
 }  // namespace
 }  // namespace lpbf_amd
 
@@ -674,16 +685,33 @@ void lpbf_launch_step_ex_batch_flush(const double* T, int nx, int ny, int j,
     check_launch();
 }
 
+int lpbf_step_ensemble_rows(int nx, int ny, int n_runs) {
+    if (!shape_ok(nx, ny) || n_runs < 1) return 0;
+    // measured on 240x120 rows (64 and 256 of them): short bands pay the
+    // per-wavefront set-up and reductions, few wavefronts leave SIMDs idle;
+    // 4-row bands up to ~8k wavefronts in all, longer ones beyond
+    const long long nwx = (nx + kBlockCols - 1) / kBlockCols;
+    int R = 4;
+    while (R < 32 && nwx * ((ny + R - 1) / R) * n_runs > 8192) R *= 2;
+    return R;
+}
+
+int lpbf_step_ensemble_nblocks(int nx, int ny, int rows) {
+    if (!shape_ok(nx, ny) || rows < 0 || rows > kMaxEnsembleRows) return 0;
+    return grid_of(ensemble_region(nx, ny, rows, BC_MIRROR, 0.0, 0.0));
+}
+
 void lpbf_launch_step_ensemble(const lpbf_ensemble_run* d_runs, int n_runs, int nx, int ny,
-                               int phys, int bc, double T_bc, double Tm, const int* d_s0, int j,
-                               hipStream_t stream) {
-    if (!d_runs || n_runs < 1 || n_runs > 65535 || !shape_ok(nx, ny) || !d_s0 || j < 0 ||
+                               int rows, int phys, int bc, double T_bc, double Tm,
+                               const int* d_s0, int j, hipStream_t stream) {
+    const int nblk = lpbf_step_ensemble_nblocks(nx, ny, rows);
+    if (!d_runs || n_runs < 1 || n_runs > 65535 || nblk <= 0 || !d_s0 || j < 0 ||
         (phys != 0 && phys != 1) || (bc != LPBF_BC_MIRROR && bc != LPBF_BC_DIRICHLET) ||
-        static_cast<long long>(lpbf_step_ex_batch_nblocks(nx, ny) + 1) * n_runs > (1LL << 31) - 1) {
+        static_cast<long long>(nblk + 1) * n_runs > (1LL << 31) - 1) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }
-    Region g = plan_region(nullptr, nullptr, nx, ny, bc, T_bc, Tm);
+    Region g = ensemble_region(nx, ny, rows, bc, T_bc, Tm);
     g.step = const_cast<int*>(d_s0);
     StepConst k{};
     launch_step<RED_ENSEMBLE>(g, k, phys, const_cast<lpbf_ensemble_run*>(d_runs), j, stream,
@@ -691,13 +719,14 @@ void lpbf_launch_step_ensemble(const lpbf_ensemble_run* d_runs, int n_runs, int 
 }
 
 void lpbf_launch_step_ensemble_flush(const lpbf_ensemble_run* d_runs, int n_runs, int nx, int ny,
-                                     const int* d_s0, int j, hipStream_t stream) {
-    if (!d_runs || n_runs < 1 || n_runs > 65535 || !shape_ok(nx, ny) || !d_s0 || j < 1) {
+                                     int rows, const int* d_s0, int j, hipStream_t stream) {
+    const int nblk = lpbf_step_ensemble_nblocks(nx, ny, rows);
+    if (!d_runs || n_runs < 1 || n_runs > 65535 || nblk <= 0 || !d_s0 || j < 1) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }
     hipLaunchKernelGGL(k_ensemble_flush, dim3(n_runs), dim3(kThreads), 0, stream, d_runs, d_s0,
-                       j, lpbf_step_ex_batch_nblocks(nx, ny), nx, ny);
+                       j, nblk, nx, ny);
     check_launch();
 }
 

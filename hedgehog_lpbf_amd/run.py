@@ -114,7 +114,7 @@ class Ensemble:
     ticks), and optionally x_csv, wrapped (cruise gate) and Tamb."""
 
     def __init__(self, runs, *, Tm: float, Lx: float, Ly: float, phys: int = 0, bc: int = 0,
-                 T_bc: float = 300.0, batch: int = 64, keep_records: bool = True,
+                 T_bc: float = 300.0, batch: int = 64, rows: int = 0, keep_records: bool = True,
                  graph: bool = True):
         n = len(runs)
         ks = (StepConst * n)(*[r["k"] for r in runs])
@@ -131,7 +131,7 @@ class Ensemble:
         tamb = np.ascontiguousarray([float(r.get("Tamb", 300.0)) for r in runs])
         steps = np.ascontiguousarray(lens, dtype=np.int32)
         self._keep = [ks, plan, xc, wr, tamb, steps]
-        opt = RunOptions(64, bc, T_bc, batch, 4, 0, -1.0, int(keep_records), int(graph), 0)
+        opt = RunOptions(64, bc, T_bc, batch, 4, rows, -1.0, int(keep_records), int(graph), 0)
         h = C.c_void_p()
         rc = lib().lpbf_ensemble_create(
             ks, n, phys, Tm, Lx, Ly, tamb.ctypes.data, plan.ctypes.data if plan.size else None,

@@ -8,9 +8,10 @@
 // statistics, the per-step merge into a device ring, on cruise steps the
 // hot-column copy into a column ring, and on dump steps an asynchronous copy
 // of the field to host memory; then one device-to-host copy of the rings
-// (pinned staging) and one stream synchronisation, after which the per-step
-// metrics (W, W_sub, running maxima, the cruise-gated steady W_sub) are
-// finalised on the host in step order, exactly as MetricsTask does.
+// into pinned staging.  The next batch is queued before the host waits for
+// that copy and finalises the per-step metrics (W, W_sub, running maxima,
+// the cruise-gated steady W_sub) in step order, exactly as MetricsTask
+// does, so the GPU does not wait for the host.
 //
 // Precision 64 runs the FP64 parity kernels, one step per launch, so every
 // per-step value equals the reference's.  A full batch that starts at a
@@ -112,7 +113,18 @@ public:
     int graph_launches() const noexcept { return graph_launches_; }
 
 private:
-    void batch(int bn);
+    struct Snap {
+        int step;
+        std::vector<double> field;
+    };
+    // a queued batch: steps [s0, s0 + bn), pinned half `slot`
+    struct Pending {
+        int s0 = 0, bn = 0, slot = 0;
+        std::vector<char> has_col;   // batch step copied a hot column
+        std::vector<Snap> snaps;     // snapshot copies in flight
+    };
+    Pending enqueue(int s0, int bn, int slot);
+    void complete(Pending& p);
     bool graph_batch(int s0, int bn);
     void capture(int parity);
     void release() noexcept;
@@ -139,8 +151,9 @@ private:
     StepStats* d_ring_ = nullptr;  // per-step stats of a batch
     double* d_cols_ = nullptr;     // per-step hot columns of a batch
     double* d_snap_ = nullptr;     // FP32 path: FP64 staging for snapshots
-    StepStats* h_ring_ = nullptr;  // pinned
-    double* h_cols_ = nullptr;     // pinned
+    StepStats* h_ring_ = nullptr;  // pinned, two halves
+    double* h_cols_ = nullptr;     // pinned, two halves
+    void* done_ev_[2] = {nullptr, nullptr};   // a half's copies have landed
     // FP64 graph replay: laser ticks and cruise flags of the whole run, the
     // current batch's copies the captured launches read, one captured batch
     // per physical buffer holding the current field

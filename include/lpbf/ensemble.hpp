@@ -10,14 +10,17 @@
 // HBM, and batch step j of the ensemble is one lpbf_launch_step_ensemble
 // over all runs (per run: the step's compute workgroups plus one that
 // finishes the run's previous step into its ring).  A batch is `batch`
-// such launches and a flush, replayed as one captured hipGraph; one copy
-// of all rings (and of the hot columns when a cruise step is in the batch)
-// to pinned memory and one synchronisation end it, and the host finalises
-// each run's metrics in step order with the same RunMetrics as DeviceRun.
+// such launches and a flush, replayed as one captured hipGraph, then one
+// copy of all rings (and of the hot columns when a cruise step is in the
+// batch) to pinned memory.  The next batch is queued before the host waits
+// for this one and finalises each run's metrics in step order with the same
+// RunMetrics as DeviceRun.
 //
-// Every cell update, block merge and hot-column copy is the one DeviceRun's
-// graph batches make, so fields, per-step statistics, W, W_sub and the
-// steady W_sub equal a DeviceRun of the same run bit for bit.  Runs may
+// Every cell update and hot-column copy is the one DeviceRun makes, so
+// fields, the integer per-step statistics, tmax, W, W_sub and the steady
+// W_sub equal a DeviceRun of the same run bit for bit; the FP64 energy sums
+// merge bands of another height (chosen for the whole ensemble) and may
+// differ in their last bits.  Runs may
 // differ in their constants (P, V, r0, eta, dSub, kmult, dt, h), plan
 // length and initial field; the grid, physics, boundary condition and Tm
 // are shared.  No snapshots (the manifest loop writes none).
@@ -42,6 +45,7 @@ struct EnsembleOptions {
     int bc = LPBF_BC_MIRROR;
     double T_bc = 300.0;      // Dirichlet edge temperature
     int batch = 64;           // steps per host synchronisation
+    int rows = 0;             // band height per wavefront (0: lpbf_step_ensemble_rows)
     bool keep_records = false;
     bool graph = true;        // replay full batches as one captured hipGraph
 };
@@ -71,10 +75,17 @@ public:
     std::vector<StepRecord> const& records(int m) const { return rec_.at(static_cast<std::size_t>(m)); }
     int graph_launches() const noexcept { return graph_launches_; }
     void* stream() const noexcept { return stream_; }
+    int rows() const noexcept { return opt_.rows; }     // band height in use
 
 private:
+    // a queued batch: steps [s0, s0 + bn), host buffers `slot`, hot columns
+    // copied for batch steps [c_lo, c_hi]
+    struct Pending {
+        int s0, bn, slot, c_lo, c_hi;
+    };
     EnsembleMember const& at(int m) const { return runs_.at(static_cast<std::size_t>(m)); }
-    void batch(int bn);
+    Pending enqueue(int s0, int bn, int slot);
+    void complete(Pending const& p);
     void launches(int bn);
     void capture();
     void release() noexcept;
@@ -90,11 +101,12 @@ private:
     void* d_runs_ = nullptr;          // lpbf_ensemble_run[n] (device)
     std::vector<double*> buf_;        // 2 per run
     int* d_s0_ = nullptr;             // batch start step (device)
-    int* h_s0_ = nullptr;             // pinned
+    int* h_s0_ = nullptr;             // pinned, two
     StepStats* d_ring_ = nullptr;     // [n][batch]
     double* d_cols_ = nullptr;        // [batch][n][ny] (step-major)
-    StepStats* h_ring_ = nullptr;     // pinned
-    double* h_cols_ = nullptr;        // pinned
+    StepStats* h_ring_ = nullptr;     // pinned, two halves
+    double* h_cols_ = nullptr;        // pinned, two halves
+    void* done_ev_[2] = {nullptr, nullptr};   // a half's copies have landed
     void* graph_ = nullptr;
     void* gexec_ = nullptr;
     std::vector<RunMetrics> met_;

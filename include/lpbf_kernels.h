@@ -138,13 +138,16 @@ void lpbf_launch_step_ex_batch_flush(const double* T, int nx, int ny, int j,
  * run has its own constants, two field buffers (step s reads buf[s & 1] and
  * writes the other, so after n steps the field is in buf[n & 1]), whole-run
  * laser plan and cruise flags, 2 * nblk block records (nblk =
- * lpbf_step_ex_batch_nblocks(nx, ny)), and a ring and column buffer for one
- * batch.  Launch j computes step s = *d_s0 + j of every run still running
- * (*d_s0 = the batch's first step, in device memory, set by the caller
- * before the batch) and finishes step s - 1 into ring[j-1] and, on cruise
- * steps, cols[(j-1)*col_pitch ...] (ny values), exactly as lpbf_launch_step_ex_batch; runs
- * whose n_steps are reached sit out.  The flush (j = batch length) finishes
- * the batch's last step.  phys, bc, T_bc and Tm are shared. */
+ * lpbf_step_ensemble_nblocks(nx, ny, rows)), and a ring and column buffer
+ * for one batch.  Launch j computes step s = *d_s0 + j of every run still
+ * running (*d_s0 = the batch's first step, in device memory, set by the
+ * caller before the batch) and finishes step s - 1 into ring[j-1] and, on
+ * cruise steps, cols[(j-1)*col_pitch ...] (ny values), exactly as
+ * lpbf_launch_step_ex_batch; runs whose n_steps are reached sit out.  The
+ * flush (j = batch length) finishes the batch's last step.  rows is the
+ * band height of a wavefront (1..1024; 0 = lpbf_launch_step_ex_batch's, so
+ * the block records and their sums equal a single run's); phys, bc, T_bc
+ * and Tm are shared. */
 typedef struct lpbf_ensemble_run {
     StepConst            k;        /* the run's constants; x_l, y_l, q_scale unused */
     double*              buf[2];   /* field buffers (device) */
@@ -156,11 +159,16 @@ typedef struct lpbf_ensemble_run {
     int                  n_steps;
     int                  col_pitch; /* doubles from one step's column to the next, >= ny */
 } lpbf_ensemble_run;
+int  lpbf_step_ensemble_nblocks(int nx, int ny, int rows);   /* 0: invalid */
+/* Band height for n_runs runs of an nx x ny plate (what lpbf::Ensemble uses
+ * by default); 0 for an invalid shape. */
+int  lpbf_step_ensemble_rows(int nx, int ny, int n_runs);
 void lpbf_launch_step_ensemble(const lpbf_ensemble_run* d_runs, int n_runs, int nx, int ny,
-                               int phys, int bc, double T_bc, double Tm, const int* d_s0,
-                               int j, hipStream_t stream);
+                               int rows, int phys, int bc, double T_bc, double Tm,
+                               const int* d_s0, int j, hipStream_t stream);
 void lpbf_launch_step_ensemble_flush(const lpbf_ensemble_run* d_runs, int n_runs, int nx,
-                                     int ny, const int* d_s0, int j, hipStream_t stream);
+                                     int ny, int rows, const int* d_s0, int j,
+                                     hipStream_t stream);
 
 /* Merge block StepStats in a fixed order (deterministic FP64 sums). */
 void lpbf_launch_stats_finalize(const StepStats* parts, int nblocks, int nx,

@@ -81,13 +81,15 @@ int  lpbf_run_records(const lpbf_run* run, int first, int n, lpbf_step_record* o
  * dSub, kmult, dt, ...), laser plan, length and initial field.  Every step
  * of every run still running is one lpbf_launch_step_ensemble per step for
  * all runs; the fields, per-step records and summaries equal those of
- * lpbf_run_* runs of the same inputs bit for bit.
+ * lpbf_run_* runs of the same inputs bit for bit, except the last bits of
+ * the FP64 energy sums (bands of another height).
  *
  * plan / x_csv / wrapped hold the runs' ticks back to back (run m's start
  * after the n_steps of runs 0..m-1), with the meaning of lpbf_run_create's;
  * Tamb[m] is run m's initial uniform field (NULL: 300 K).  opt: precision
- * must be 64; bc, T_bc, batch, keep_records and graph are used.  Snapshots
- * are not taken.  Calls on one handle must not overlap. */
+ * must be 64; bc, T_bc, batch, rows (band height of the ensemble launch,
+ * 0..1024, 0 = lpbf_step_ensemble_rows), keep_records and graph are used.
+ * Snapshots are not taken.  Calls on one handle must not overlap. */
 typedef struct lpbf_ensemble lpbf_ensemble;
 int  lpbf_ensemble_create(const StepConst* k, int n_runs, int phys, double Tm, double Lx,
                           double Ly, const double* Tamb, const LaserTick* plan,

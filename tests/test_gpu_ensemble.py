@@ -2,12 +2,12 @@
 and lpbf_gpu --manifest --ensemble) against independent DeviceRuns of the
 same inputs.
 
-Every cell update, block merge and hot-column copy of an ensemble run is the
-one a DeviceRun graph batch makes, so fields, per-step integer statistics,
-tmax, W, W_sub and the run summaries must be identical (bitwise).  The FP64
-energy sums of a DeviceRun's eager (non-graph) batches merge a different
-number of block records, so against DeviceRun they are compared to 1e-12
-relative; between two ensembles they are bitwise too."""
+Every cell update and hot-column copy of an ensemble run is the one a
+DeviceRun makes, so fields, per-step integer statistics, tmax, W, W_sub and
+the run summaries must be identical (bitwise).  The FP64 energy sums merge
+bands of another height (the ensemble picks its own), so against DeviceRun
+they are compared to 1e-12 relative; between two ensembles with the same
+band height they are bitwise too."""
 import os
 import subprocess
 
@@ -136,3 +136,23 @@ def test_lpbf_gpu_manifest_ensemble_equals_row_loop(tmp_path):
     bad = subprocess.run([EXE, "--manifest=%s" % man, "--ensemble", "--precision=32"],
                          capture_output=True, text=True, timeout=120)
     assert bad.returncode == 1 and "--ensemble" in bad.stderr
+
+
+@pytest.mark.parametrize("rows", [1, 3, 120])
+def test_ensemble_band_heights_give_the_same_runs(rows):
+    print(f"\n[case] Ensemble of 3 runs, band height {rows} vs the automatic one", flush=True)
+    built = [_member(r, 1, 2e-4) for r in ROWS[:3]]
+    runs = [b[0] for b in built]
+    mat, dom = built[0][1], built[0][2]
+    out = []
+    for rw in (0, rows):
+        with h.Ensemble(runs, Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, phys=1, batch=16,
+                        rows=rw) as ens:
+            ens.advance(10 ** 6)
+            out.append(([ens.download(i) for i in range(3)], [ens.records(i) for i in range(3)]))
+    (Ta, ra), (Tb, rb) = out
+    for i in range(3):
+        assert np.array_equal(Ta[i], Tb[i])
+        assert [_ints(x) for x in ra[i]] == [_ints(y) for y in rb[i]]
+        for x, y in zip(ra[i], rb[i]):
+            assert np.all(np.abs(_sums(x) - _sums(y)) <= 1e-12 * (np.abs(_sums(y)) + 1.0) * 28800)
