@@ -1,8 +1,8 @@
 """FP32 performance path: invariances, statistics and error against FP64.
 
 The FP32 kernel is not a bit-parity path; its bar is the error against the
-FP64 oracle, stated per test (the reference north star asks for <= 0.5 K at
-the full 8192^2 x 50k-step workload; these small runs get tighter bounds).
+FP64 oracle, stated per test (the FP32 target is <= 0.5 K at the full
+8192^2 x 50k-step workload; these small runs get tighter bounds).
 Exact checks: the field is bitwise independent of temporal depth and band
 height (the blocking changes where operands come from, not the arithmetic),
 and the fused statistics equal a host reduction of the field they describe.
