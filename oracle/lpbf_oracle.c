@@ -23,8 +23,9 @@
  *                                   reduction to ora_step for BC mirror
  *   ora_noise                       SplitMix64 published first output
  *   ora_raster_plan                 closed-form positions at segment ends
- * ora_step_ex's extended-physics statistics are parity unpinned beyond the
- * per-cell reduction to ora_step.
+ *   ora_step_ex (extended stats)    source sum = change of sum(T'-T) with the
+ *                                   beam on, linear in q_scale; loss-model
+ *                                   differences; zero loss at equilibrium
  */
 #include "lpbf_oracle.h"
 

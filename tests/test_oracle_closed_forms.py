@@ -166,6 +166,61 @@ def test_energy_identity_mirror_extended(ora):
     assert abs((T1 - T).sum() - st.sum_dT) < 1e-9 * abs(st.sum_dT)
 
 
+def test_extended_energy_split(ora):
+    """The extended statistics' source and loss sums, pinned from the field.
+
+    Per cell T' = T + dt*(div + S - loss)/(rho*cp_eff(T)*...) with cp_eff,
+    div and loss depending on the old field only, so for one field:
+      * the source sum is linear in q_scale and equals the change of sum(T'-T)
+        when the beam is switched on;
+      * the loss sums of two loss models differ by minus the change of
+        sum(T'-T) (convection + radiation doubled, then the substrate
+        conductance doubled): a dropped or mis-weighted loss term fails;
+      * at T = Tamb = Tsub (uniform) every loss vanishes and div = 0, so
+        sum_loss = 0 and sum_dT = sum_src.
+    The cells under the beam and a quarter of the rest lie in the latent
+    window (1533-1609 K), where cp_eff carries the latent heat, so a source
+    or loss sum weighted by cp instead of cp_eff fails."""
+    s = _setup(ora, 40, 30)
+    k = ora.make_const(s, ora.dt_cfl_ext(s, 1.5))
+    k.kmult = 1.5
+    k.x_l, k.y_l = 17.3 * k.dx, 12.6 * k.dy
+    rng = np.random.default_rng(5)
+    T = 300 + 2000 * rng.random((30, 40))
+    latent = 1540.0 + 60.0 * rng.random((30, 40))
+    T = np.where(rng.random((30, 40)) < 0.25, latent, T)
+    T[8:18, 12:24] = latent[8:18, 12:24]            # under the beam
+    scale = np.abs(T).sum() * 1e-13
+
+    def run(**kw):
+        kk = ora.make_const(s, k.dt)
+        C.pointer(kk)[0] = k
+        for name, v in kw.items():
+            setattr(kk, name, v)
+        T1, st = ora.step_ex(T, kk, phys=1)
+        assert abs((T1 - T).sum() - st.sum_dT) <= scale
+        return st
+
+    on, off, two = run(), run(q_scale=0.0), run(q_scale=2.0)
+    assert off.sum_src == 0.0 and on.sum_src > 0.0
+    assert abs(on.sum_src - (on.sum_dT - off.sum_dT)) <= scale
+    assert abs(two.sum_src - 2.0 * on.sum_src) <= scale
+    assert on.sum_loss == off.sum_loss == two.sum_loss
+    more_top = run(hConv=2.0 * k.hConv, eps=2.0 * k.eps)
+    more_bot = run(dSub=0.5 * k.dSub)
+    for st in (more_top, more_bot):
+        assert st.sum_loss > on.sum_loss
+        assert abs((st.sum_loss - on.sum_loss) + (st.sum_dT - on.sum_dT)) <= scale
+    # uniform field at the ambient and substrate temperature: no loss, no diffusion
+    Tu = np.full((30, 40), 700.0)
+    ku = ora.make_const(s, k.dt)
+    C.pointer(ku)[0] = k
+    ku.Tamb = ku.Tsub = 700.0
+    T1, st = ora.step_ex(Tu, ku, phys=1)
+    assert st.sum_loss == 0.0
+    assert abs(st.sum_dT - st.sum_src) <= 1e-12 * st.sum_src
+
+
 def test_source_normalisation(ora):
     """sum(peak_S * exp(-2 r^2/r0^2)) dx dy = eta * P for a resolved Gaussian."""
     s = _setup(ora, 200, 160)
