@@ -19,8 +19,9 @@ times the row-by-row loop beside it.
 Prints one JSON line: value = cell-updates/s over the timed steps, with the
 inputs already resident in HBM; `roofline` for the dominant kernel (step
 kernel duration from HIP events on its stream); `cpu_baseline` = the CPU
-oracle (FP64, all host threads on row ranges) timed on a bounded sample of
-the same plate.
+restatement of the reference step in the GPU path's precision (FP32 for the
+FP32 plate, the FP64 oracle otherwise; all host threads on row ranges)
+timed on a bounded sample of the same plate.
 """
 from __future__ import annotations
 
@@ -59,15 +60,18 @@ def cpu_threads():
     return max(1, min(16, n or 1))
 
 
-def cpu_baseline(k, nx, ny, target_s=12.0):
-    """The CPU oracle (FP64 restatement of the reference step) on the same plate,
-    rows split over the host threads as the reference splits its plate into
-    row strips over a thread pool (src/main.cpp:316-348)."""
+def cpu_baseline(k, nx, ny, fp32=False, target_s=12.0):
+    """The CPU restatement of the reference step on the same plate, in the
+    precision of the GPU path it stands beside (the FP64 oracle, or its FP32
+    form), rows split over the host threads as the reference splits its
+    plate into row strips over a thread pool (src/main.cpp:316-348)."""
     from concurrent.futures import ThreadPoolExecutor
 
     import numpy as np
     from oracle import pyoracle as ora
     T = 300.0 + ora.noise_field(0, 0.0, 0.5, (ny, nx))
+    if fp32:
+        T = np.ascontiguousarray(T, dtype=np.float32)
     kk = ora.StepConst.from_buffer_copy(bytes(k))
     out = np.empty_like(T)
     nt = cpu_threads()
@@ -75,7 +79,10 @@ def cpu_baseline(k, nx, ny, target_s=12.0):
     with ThreadPoolExecutor(nt) as pool:
         t0 = time.perf_counter()
         while True:
-            ora.step_threads(T, kk, 0, threads=nt, out=out, pool=pool)
+            if fp32:
+                ora.step_threads_f32(T, kk, threads=nt, out=out, pool=pool)
+            else:
+                ora.step_threads(T, kk, 0, threads=nt, out=out, pool=pool)
             T, out = out, T
             steps += 1
             el = time.perf_counter() - t0
@@ -83,8 +90,8 @@ def cpu_baseline(k, nx, ny, target_s=12.0):
                 break
     return {"value": nx * ny * steps / el, "unit": "cell-updates/s", "cores": nt,
             "kind": "port",
-            "sample": f"oracle FP64 step (C, -O3) on {nt} host threads (row ranges), "
-                      f"{nx}x{ny} plate, {steps} steps, {el:.1f} s"}
+            "sample": f"oracle {'FP32' if fp32 else 'FP64'} step (C, -O3) on {nt} host "
+                      f"threads (row ranges), {nx}x{ny} plate, {steps} steps, {el:.1f} s"}
 
 
 def traffic_for(workload, **launch):
@@ -181,11 +188,11 @@ def batch_bench(args):
             t0 = time.perf_counter()
             for sp in specs:
                 kk = ora.StepConst.from_buffer_copy(bytes(sp.k))
-                T = np.full((sp.k.ny, sp.k.nx), 300.0)
+                T = np.full((sp.k.ny, sp.k.nx), 300.0, dtype=np.float32)
                 out = np.empty_like(T)
                 for q in range(10):
                     kk.x_l, kk.y_l, kk.q_scale = sp.plan[q]
-                    ora.step_threads(T, kk, 0, threads=nt, out=out, pool=pool)
+                    ora.step_threads_f32(T, kk, threads=nt, out=out, pool=pool)
                     T, out = out, T
                 done += 10 * sp.k.nx * sp.k.ny
                 n_inst += 1
@@ -193,7 +200,7 @@ def batch_bench(args):
                     break
             el = time.perf_counter() - t0
         cpu = {"value": done / el, "unit": "cell-updates/s", "cores": nt, "kind": "port",
-               "sample": f"oracle FP64 step (C, -O3) on {nt} host threads, first {n_inst} "
+               "sample": f"oracle FP32 step (C, -O3) on {nt} host threads, first {n_inst} "
                          f"instances x 10 steps, {el:.1f} s"}
     achieved = 8.0 * cells * steps / (total_ms * 1e-3) / 1e9
     out = {
@@ -623,7 +630,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(k, nx, ny)
+        cpu = cpu_baseline(k, nx, ny, fp32=args.workload == "plate8k_f32")
 
     out = {
         "metric": "cell-updates/s",

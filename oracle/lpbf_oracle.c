@@ -26,6 +26,9 @@
  *   ora_step_ex (extended stats)    source sum = change of sum(T'-T) with the
  *                                   beam on, linear in q_scale; loss-model
  *                                   differences; zero loss at equilibrium
+ *   ora_step_rows_f32               (the FP32 CPU baseline, no parity role)
+ *                                   within FP32 rounding of ora_step on a
+ *                                   field whose every term is far above it
  */
 #include "lpbf_oracle.h"
 
@@ -249,6 +252,50 @@ void ora_step_rows(const double* Tin, double* Tout, const StepConst* k, int phys
             } else {
                 Tout[c] = cell_ext(T, Txm, Txp, Tym, Typ, xpos, ypos, k, &src, &lossv);
             }
+        }
+    }
+}
+
+/* ora_step_rows in FP32 (baseline physics, mirror edges): the reference
+ * expression (step_task.hpp:93-129) evaluated in float storage and
+ * arithmetic with expf.  Not a parity function: it is the FP32 CPU baseline
+ * the benchmark times beside the FP32 GPU path. */
+void ora_step_rows_f32(const float* Tin, float* Tout, const StepConst* k, int y0, int y1) {
+    const int nx = k->nx, ny = k->ny;
+    const float inv_dx2 = (float)k->inv_dx2, inv_dy2 = (float)k->inv_dy2;
+    const float alpha = (float)k->alpha, dt = (float)k->dt;
+    const float dx = (float)k->dx, dy = (float)k->dy;
+    const float x_l = (float)k->x_l, y_l = (float)k->y_l, inv_r0_sq = (float)k->inv_r0_sq;
+    const float S0 = (float)(k->q_scale * k->peak_S / (k->rho_cp * k->h));
+    const float hConv = (float)k->hConv, Tamb = (float)k->Tamb, Tsub = (float)k->Tsub;
+    const float eps_sb = (float)(k->eps * ORA_SIGMA_SB);
+    const float Ta4 = Tamb * Tamb * Tamb * Tamb;
+    const float Kd = (float)(k->K / k->dSub);
+    const float loss_w = (float)(k->inv_h / k->rho_cp);
+    const int rad = k->radiation;
+    if (y0 < 0) y0 = 0;
+    if (y1 > ny) y1 = ny;
+    for (int y = y0; y < y1; ++y) {
+        const float ypos = (float)y * dy;
+        for (int x = 0; x < nx; ++x) {
+            const long long c = (long long)y * nx + x;
+            const float T = Tin[c];
+            const float xpos = (float)x * dx;
+            const float Txm = (x == 0) ? T : Tin[c - 1];
+            const float Txp = (x == nx - 1) ? T : Tin[c + 1];
+            const float Tym = (y == 0) ? T : Tin[c - nx];
+            const float Typ = (y == ny - 1) ? T : Tin[c + nx];
+            float lap_x;
+            if (x == 0)           lap_x = (Txp - T) * inv_dx2;
+            else if (x == nx - 1) lap_x = (Txm - T) * inv_dx2;
+            else                  lap_x = (Txp - 2.0f * T + Txm) * inv_dx2;
+            const float lap_y = (Typ - 2.0f * T + Tym) * inv_dy2;
+            const float r2 = (xpos - x_l) * (xpos - x_l) + (ypos - y_l) * (ypos - y_l);
+            const float S = S0 * expf(-2.0f * r2 * inv_r0_sq);
+            float q_top = hConv * (T - Tamb);
+            if (rad) q_top += eps_sb * (T * T * T * T - Ta4);
+            const float loss = (q_top + Kd * (T - Tsub)) * loss_w;
+            Tout[c] = T + dt * (alpha * (lap_x + lap_y) + S - loss);
         }
     }
 }

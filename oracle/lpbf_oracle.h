@@ -86,6 +86,9 @@ void   ora_step(const double* Tin, double* Tout, const StepConst* k, int phys);
  * several host threads. */
 void   ora_step_rows(const double* Tin, double* Tout, const StepConst* k, int phys,
                      int y0, int y1);
+/* The baseline step of ora_step_rows in FP32 storage and arithmetic (the
+ * FP32 CPU baseline; mirror edges). */
+void   ora_step_rows_f32(const float* Tin, float* Tout, const StepConst* k, int y0, int y1);
 
 /* Row-strip tile step (step_task.hpp:54-206).  `tile` holds rows+2 rows of
  * nx values: halo row 0, owned rows 1..rows, halo row rows+1.  Owned rows

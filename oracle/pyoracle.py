@@ -111,6 +111,7 @@ def load() -> C.CDLL:
         "ora_make_const": (v, [C.c_void_p, C.c_void_p, C.c_void_p, d, C.c_void_p]),
         "ora_step": (v, [_DP, _DP, C.c_void_p, i]),
         "ora_step_rows": (v, [_DP, _DP, C.c_void_p, i, i, i]),
+        "ora_step_rows_f32": (v, [C.c_void_p, C.c_void_p, C.c_void_p, i, i]),
         "ora_step_ex": (v, [_DP, _DP, C.c_void_p, i, i, d, d, C.c_void_p]),
         "ora_step_tile": (v, [_DP, _DP, i, i, C.c_void_p, i]),
         "ora_step_tiled": (v, [_DP, _DP, C.c_void_p, i, i]),
@@ -180,6 +181,27 @@ def make_const(s: Setup, dt: float) -> StepConst:
 def step(T: np.ndarray, k: StepConst, phys: int = 0) -> np.ndarray:
     out = np.empty_like(T)
     lib().ora_step(_dp(T), _dp(out), C.byref(k), phys)
+    return out
+
+
+def step_threads_f32(T: np.ndarray, k: StepConst, threads: int = 1,
+                     out: np.ndarray | None = None, pool=None) -> np.ndarray:
+    """ora_step_rows_f32 (baseline physics, mirror edges) on a float32 field,
+    rows split over `threads` host threads."""
+    assert T.dtype == np.float32 and T.flags.c_contiguous
+    out = np.empty_like(T) if out is None else out
+    ny = T.shape[0]
+    cuts = [ny * t // threads for t in range(threads + 1)]
+    pT, pO, pk = T.ctypes.data, out.ctypes.data, C.byref(k)
+
+    def part(t):
+        lib().ora_step_rows_f32(pT, pO, pk, cuts[t], cuts[t + 1])
+
+    if pool is None or threads == 1:
+        for t in range(threads):
+            part(t)
+    else:
+        list(pool.map(part, range(threads)))
     return out
 
 

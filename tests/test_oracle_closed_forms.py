@@ -431,3 +431,25 @@ def test_threaded_rows_equal_single_step(ora):
     with ThreadPoolExecutor(5) as pool:
         got = ora.step_threads(T, k, 0, threads=5, pool=pool)
     assert np.array_equal(got, ora.step(T, k, 0))
+
+
+def test_fp32_cpu_baseline_step_within_fp32_rounding(ora):
+    """The FP32 CPU baseline step (timed beside the FP32 GPU path) against the
+    FP64 oracle, on a field where diffusion, source, convection, radiation and
+    substrate loss each move a cell by far more than FP32 rounding (edges
+    included), so a dropped or wrong term fails the bound."""
+    s = _setup(ora, 64, 48)
+    s.proc.hConv = 1e5
+    s.mat.eps = 1.0
+    k = ora.make_const(s, ora.dt_cfl(s))
+    k.x_l, k.y_l = 31.4 * k.dx, 22.7 * k.dy
+    rng = np.random.default_rng(11)
+    T = 300 + 2000 * rng.random((48, 64))
+    ref = ora.step(T, k)
+    got = ora.step_threads_f32(np.ascontiguousarray(T, dtype=np.float32), k, threads=3)
+    # FP32 rounding of values up to ~3000 K over a dozen operations
+    assert np.abs(got - ref).max() <= 5e-3
+    # every cell moves by 100x the bound or more (the planted slips checked on
+    # the CPU: radiation, substrate loss or source dropped, lap_y scaled
+    # by 1.001, each fail the first bound)
+    assert np.abs(ref - T).min() > 0.5
