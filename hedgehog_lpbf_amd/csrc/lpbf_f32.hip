@@ -7,11 +7,12 @@
 namespace lpbf_amd {
 namespace {
 
-template <int K, int BC, bool RAD>
+template <int K, int BC, bool RAD, bool RAGGED>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == 4 && !RAD ? 2 : 3))) void k_step_f32(F32Args a,
                                                        const LaserTick* __restrict__ plan,
                                                        int step0) {
-    step_tile<K, BC, RAD>(a, nullptr, nullptr, plan, step0, xcd_remap(blockIdx.x, a.nwg));
+    step_tile<K, BC, RAD, false, RAGGED>(a, nullptr, nullptr, plan, step0,
+                                         xcd_remap(blockIdx.x, a.nwg));
 }
 
 // One workgroup per level merges that level's workgroup records.
@@ -63,14 +64,21 @@ __global__ void k_T_to_theta(const double* __restrict__ T, int nx, int ny, doubl
     }
 }
 
+template <int K, bool RAD, bool RAGGED>
+void launch_krg(const F32Args& a, const LaserTick* plan, int step0, hipStream_t s) {
+    if (a.bc == LPBF_BC_DIRICHLET)
+        hipLaunchKernelGGL((k_step_f32<K, LPBF_BC_DIRICHLET, RAD, RAGGED>), dim3(a.nwg),
+                           dim3(kThreads), 0, s, a, plan, step0);
+    else
+        hipLaunchKernelGGL((k_step_f32<K, LPBF_BC_MIRROR, RAD, RAGGED>), dim3(a.nwg),
+                           dim3(kThreads), 0, s, a, plan, step0);
+}
+
+// A plate whose width is a multiple of 4 has no lane straddling column nx.
 template <int K, bool RAD>
 void launch_kr(const F32Args& a, const LaserTick* plan, int step0, hipStream_t s) {
-    if (a.bc == LPBF_BC_DIRICHLET)
-        hipLaunchKernelGGL((k_step_f32<K, LPBF_BC_DIRICHLET, RAD>), dim3(a.nwg), dim3(kThreads), 0,
-                           s, a, plan, step0);
-    else
-        hipLaunchKernelGGL((k_step_f32<K, LPBF_BC_MIRROR, RAD>), dim3(a.nwg), dim3(kThreads), 0, s,
-                           a, plan, step0);
+    if (a.nx % 4 == 0) launch_krg<K, RAD, false>(a, plan, step0, s);
+    else launch_krg<K, RAD, true>(a, plan, step0, s);
 }
 
 template <int K>

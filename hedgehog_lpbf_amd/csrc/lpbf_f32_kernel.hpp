@@ -123,7 +123,10 @@ struct LevelAcc {
 // where its fields are loaded at their uses (a local copy of `a` costs nine
 // more VGPRs and one wave per SIMD at K = 4; the kernel-argument form is
 // allocated best reading a.in / a.out itself).
-template <int K, int BC, bool RAD, bool BUFS = false>
+// RAGGED = false promises nx % 4 == 0: strip origins are multiples of 4, so
+// no lane then holds both column nx-1 and a column >= nx, and the masking of
+// such columns out of the statistics drops from the code.
+template <int K, int BC, bool RAD, bool BUFS = false, bool RAGGED = true>
 __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restrict__ src,
                                           float* __restrict__ dst,
                                           const LaserTick* __restrict__ plan, int step0, int wg) {
@@ -185,7 +188,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         const bool has_last = (pL >= 0 && pL < kLoadCols);
         const int lastl = pL >> 2, laste = pL & 3;
         // the lane that holds column nx-1 may also hold columns >= nx
-        const bool ragged = has_right && pade != 0;
+        const bool ragged = RAGGED && has_right && pade != 0;
         // strips holding a domain edge column take the column fix-ups
         const bool edge = has_left || has_right || has_last;
 
@@ -323,7 +326,9 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                     // FP32 resolution of any theta
                     const float sv = cS[j - 1] * __builtin_amdgcn_exp2f(-fmaf(a.ax * ux, ux, ey));
                     out4[e] += sv;
-                    ssum += (col4 + e < nx) ? sv : 0.f;
+                    // lanes past column nx own nothing and are dropped later;
+                    // only a straddling lane needs the per-column test
+                    ssum += (!RAGGED || col4 + e < nx) ? sv : 0.f;
                 }
             }
 #pragma unroll
