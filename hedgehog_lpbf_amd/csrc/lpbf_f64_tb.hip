@@ -60,11 +60,21 @@ struct D2 {
     double x, y;
 };
 
-template <int K, int PHYS, int BC>
+// EVEN = true promises an even nx with mirror edges: lane columns start at
+// even positions, so a lane's two columns are both inside the field or both
+// outside it, and with mirror edges every owned cell counts in the sums.
+// Ownership is then one flag per lane, applied once before the reductions;
+// the per-cell statistics run unpredicated, the running maximum takes the
+// row pair's larger value (the first column on ties, as the strict > of the
+// cell-by-cell order) and the molten count is skipped for rows below Tm.
+// At depth 3 this form needs more VGPRs than three waves per SIMD leave and
+// spills (measured 5% slower); depth 3 keeps the per-cell form.
+template <int K, int PHYS, int BC, bool EVEN>
 __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <= 3 ? 3 : 2))) void k_step_f64_tb(TbArgs a, StepConst k,
                                                             const LaserTick* __restrict__ plan,
                                                             int step0) {
     using Sh = TbShape<K>;
+    constexpr bool kLaneOwned = EVEN && BC != LPBF_BC_DIRICHLET;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int wg = xcd_remap(blockIdx.x, a.nwg);
@@ -95,6 +105,7 @@ __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <=
         s_dT[j] = s_src[j] = s_loss[j] = 0.0;
     }
     int xcol[2] = {0, 0};     // the lane's two columns (for the index)
+    bool lane_owns = false;   // kLaneOwned: the lane's cells count
 
     if (task < a.ntasks) {
         const int band = task / a.nstrips, strip = task % a.nstrips;
@@ -103,6 +114,7 @@ __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <=
         const bool in0 = x0 >= 0 && x0 < nx, in1 = x1 >= 0 && x1 < nx;
         const bool own_lane = lane >= Sh::ov && lane < kWave - Sh::ov;
         const bool own0 = own_lane && in0, own1 = own_lane && in1;
+        lane_owns = own0;
         const int ya = band * a.R;
         const int yb = min(ya + a.R, ny);
         const int y_lo = max(ya - K, 0);
@@ -164,7 +176,7 @@ __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <=
                 v = cell_ext(T, Txm_g, Txp_g, Tym, Typ, xpos, ypos, far, bm[j], k, rc, src,
                              lossv);
             }
-            if (own && upd) {
+            if (kLaneOwned ? own : (own && upd)) {   // kLaneOwned: own is the row test
                 s_dT[j] += v - T;
                 s_src[j] += src;
                 s_loss[j] += lossv;
@@ -192,10 +204,22 @@ __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <=
             const bool owned_row = r >= ya && r < yb;
             D2 res;
             res.x = cell(JP{}, c.x, xl, c.y, up.x, dn.x, x0, xe0, xpos0, ypos, far, r,
-                         owned_row && own0);
+                         kLaneOwned ? owned_row : (owned_row && own0));
             res.y = cell(JP{}, c.y, c.x, xr, up.y, dn.y, x1, xe1, xpos1, ypos, far, r,
-                         owned_row && own1);
-            if (owned_row) {
+                         kLaneOwned ? owned_row : (owned_row && own1));
+            if (kLaneOwned && owned_row) {
+                constexpr int s = j - 1;
+                const double m = __builtin_fmax(res.x, res.y);
+                if (m > best[s]) {
+                    best[s] = m;
+                    bpos[s] = (res.x == m) ? 2 * r : 2 * r + 1;
+                }
+                if (m > a.Tm) {
+                    molten[s] += (res.x > a.Tm) + (res.y > a.Tm);
+                    ymin[s] = min(ymin[s], r);
+                    ymax[s] = r;
+                }
+            } else if (owned_row) {
                 constexpr int s = j - 1;
                 const bool o0 = own0 && res.x > best[s];
                 best[s] = o0 ? res.x : best[s];
@@ -243,6 +267,16 @@ __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <=
     }
 
     if (!a.part) return;
+    if (kLaneOwned && !lane_owns) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            bpos[j] = -1;
+            ymin[j] = ny;
+            ymax[j] = -1;
+            molten[j] = 0;
+            s_dT[j] = s_src[j] = s_loss[j] = 0.0;
+        }
+    }
     // per level: wavefront then workgroup merge -> a.part[j*nwg + wg]
     __shared__ PartialAcc s_pa[kTbWaves];
     __shared__ double s_sum[3][kTbWaves];
@@ -323,11 +357,16 @@ TbGeom tb_geometry(int nx, int ny, int depth, int rows) {
 template <int K, int PHYS>
 void launch_tb_k(int bc, const TbArgs& a, const StepConst& k, const LaserTick* plan, int step0,
                  hipStream_t s) {
+    // depth 3 keeps the per-cell statistics form (register pressure)
+    constexpr bool kEvenForm = K != 3;
     if (bc == LPBF_BC_DIRICHLET)
-        hipLaunchKernelGGL((k_step_f64_tb<K, PHYS, LPBF_BC_DIRICHLET>), dim3(a.nwg),
+        hipLaunchKernelGGL((k_step_f64_tb<K, PHYS, LPBF_BC_DIRICHLET, false>), dim3(a.nwg),
+                           dim3(kTbThreads), 0, s, a, k, plan, step0);
+    else if (a.nx % 2 == 0)
+        hipLaunchKernelGGL((k_step_f64_tb<K, PHYS, LPBF_BC_MIRROR, kEvenForm>), dim3(a.nwg),
                            dim3(kTbThreads), 0, s, a, k, plan, step0);
     else
-        hipLaunchKernelGGL((k_step_f64_tb<K, PHYS, LPBF_BC_MIRROR>), dim3(a.nwg),
+        hipLaunchKernelGGL((k_step_f64_tb<K, PHYS, LPBF_BC_MIRROR, false>), dim3(a.nwg),
                            dim3(kTbThreads), 0, s, a, k, plan, step0);
 }
 
