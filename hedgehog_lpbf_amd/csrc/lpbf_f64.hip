@@ -149,6 +149,11 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
     PartialAcc pa = PartialAcc::identity(nx, g.ny_glob);
     long long molten = 0;
     double s_dT = 0.0, s_src = 0.0, s_loss = 0.0;
+    // A lane meets its cells in increasing index order (row by row, x0 then
+    // x1), so a strict > keeps the smallest index among equal maxima; the
+    // position is kept as 2*row + (second column) and turned into the index
+    // once, before the reductions.
+    int bpos = -1, xs0 = 0, xs1 = 0;
 
     if (task < g.ntasks) {
         const Recips rcp_k = recips(k);
@@ -159,6 +164,8 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
         const int yb = min(ya + g.R, ny);
         const int x0 = c0 + lane, x1 = c0 + kWave + lane;
         const bool a0 = x0 < nx, a1 = x1 < nx;
+        xs0 = x0;
+        xs1 = x1;
         const int ecol = (lane == 0) ? c0 - 1 : c0 + kBlockCols;
         const bool eok = (lane == 0 && c0 > 0) || (lane == kWave - 1 && c0 + kBlockCols < nx);
         const bool tile = (g.bc == BC_TILE);
@@ -253,15 +260,20 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                 }
                 orow[x] = v;
                 if (RED != RED_NONE) {
-                    pa.take_max(v, static_cast<long long>(yg) * nx + x);
+                    if (v > pa.tmax) {
+                        pa.tmax = v;
+                        bpos = 2 * y + j;
+                    }
                     if (v > g.Tm) {
                         if (yg < pa.ymin) pa.ymin = yg;
                         if (yg > pa.ymax) pa.ymax = yg;
                         ++molten;
                     }
-                    if ((RED == RED_STATS || RED == RED_RING || RED == RED_BATCH ||
-                         RED == RED_ENSEMBLE) && upd) {
-                        s_dT += v - T;
+                    if (RED == RED_STATS || RED == RED_RING || RED == RED_BATCH ||
+                        RED == RED_ENSEMBLE) {
+                        // held Dirichlet cells have src = lossv = 0 and add
+                        // no temperature change
+                        s_dT += v - (upd ? T : v);
                         s_src += src;
                         s_loss += lossv;
                     }
@@ -273,6 +285,8 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
     }
 
     if (RED == RED_NONE) return;
+    if (bpos >= 0)
+        pa.arg = static_cast<long long>(g.y_off + (bpos >> 1)) * nx + ((bpos & 1) ? xs1 : xs0);
     __shared__ PartialAcc s_pa[kThreads / kWave];
     __shared__ double s_sum[3][kThreads / kWave];
     __shared__ long long s_mol[kThreads / kWave];
