@@ -367,7 +367,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                 }
                 A.S += static_cast<double>(row_sum(res));
                 A.SL += row_sum(L4);
-                if (src_row) A.SS += ssum;
+                A.SS += ssum;   // 0 off the source rows: an add, no select
                 // keep each running sum updated here: left alone the compiler
                 // defers the serial adds to the end of the unrolled trip and
                 // holds every phase's partial live (about 60 extra VGPRs)
