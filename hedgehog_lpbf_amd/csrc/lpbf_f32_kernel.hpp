@@ -109,8 +109,8 @@ __device__ __forceinline__ void set_elem(float4& v, int e, float t) {
 struct LevelAcc {
     float bv;        // best theta of this lane
     int bi;          // its row-major index (smallest on ties)
-    int ymin, ymax;  // molten rows
-    int molten;
+    int ymin, ymax;  // molten rows of the wavefront's owned cells (wave-uniform)
+    int molten;      // molten owned cells of the wavefront (wave-uniform)
     double S;        // sum of theta over the owned cells after this step
     float SL;        // sum of dt*loss
     float SS;        // sum of dt*S
@@ -167,6 +167,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
     // Every lane accumulates its own statistics; the ones that own nothing
     // are dropped once, before the wavefront reductions.
     bool owner = false;
+    unsigned long long own_mask = 0;   // the owning lanes (wave-uniform)
 
     if (task < a.ntasks) {
         const int band = task / a.nstrips, strip = task % a.nstrips;
@@ -175,6 +176,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         const bool ld_ok = col4 >= 0 && col4 < nx;
         const bool own_lane = lane >= 1 && lane <= kWave - 2 && col4 < nx;
         owner = own_lane;
+        own_mask = __builtin_amdgcn_ballot_w64(own_lane);
         const int ya = band * a.R;
         const int yb = min(ya + a.R, ny);
         const int y_lo = max(ya - K, 0);
@@ -359,9 +361,17 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                     const int e = (mx.x == m4) ? 0 : (mx.y == m4) ? 1 : (mx.z == m4) ? 2 : 3;
                     A.bi = r * nx + col4 + e;
                 }
-                if (m4 > a.theta_m) {
-                    A.molten += (mx.x > a.theta_m) + (mx.y > a.theta_m) + (mx.z > a.theta_m) +
-                                (mx.w > a.theta_m);
+                // molten cells and rows are counted for the wavefront from
+                // ballots, so they live in scalar registers
+                if (__builtin_amdgcn_ballot_w64(m4 > a.theta_m) & own_mask) {
+                    A.molten += __builtin_popcountll(
+                                    __builtin_amdgcn_ballot_w64(mx.x > a.theta_m) & own_mask) +
+                                __builtin_popcountll(
+                                    __builtin_amdgcn_ballot_w64(mx.y > a.theta_m) & own_mask) +
+                                __builtin_popcountll(
+                                    __builtin_amdgcn_ballot_w64(mx.z > a.theta_m) & own_mask) +
+                                __builtin_popcountll(
+                                    __builtin_amdgcn_ballot_w64(mx.w > a.theta_m) & own_mask);
                     A.ymin = min(A.ymin, r);
                     A.ymax = r;
                 }
@@ -456,7 +466,12 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
     const long long N = static_cast<long long>(nx) * ny;
     if (!owner) {
 #pragma unroll
-        for (int j = 0; j < K; ++j) acc[j] = LevelAcc{kNegInf, 0x7fffffff, ny, -1, 0, 0.0, 0.f, 0.f};
+        for (int j = 0; j < K; ++j) {
+            acc[j].bv = kNegInf;
+            acc[j].bi = 0x7fffffff;
+            acc[j].S = 0.0;
+            acc[j].SL = acc[j].SS = 0.f;
+        }
         S0 = 0.0;
     }
     double prevS = wave_sum(S0);   // lane 0 holds the wavefront sums
@@ -472,7 +487,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         const double wS = wave_sum(A.S);
         const double ws = wave_sum(static_cast<double>(A.SS));
         const double wl = wave_sum(static_cast<double>(A.SL));
-        const long long wm = wave_sum(static_cast<long long>(A.molten));
+        const long long wm = A.molten;   // already the wavefront's count
         if (lane == 0) {
             s_pa[wave] = w;
             s_sum[0][wave] = wS - prevS;   // sum of (theta' - theta) over the owned cells
