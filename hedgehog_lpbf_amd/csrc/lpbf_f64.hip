@@ -142,7 +142,8 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
         k.q_scale = t.q_scale;
     }
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
+    // wave-uniform: the block, its rows and the row loop live in SGPRs
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int task = bid * (kThreads / kWave) + wave;
     const int nx = g.nx, ny = g.ny;
 

@@ -66,9 +66,8 @@ struct D2 {
 // Ownership is then one flag per lane, applied once before the reductions;
 // the per-cell statistics run unpredicated, the running maximum takes the
 // row pair's larger value (the first column on ties, as the strict > of the
-// cell-by-cell order) and the molten count is skipped for rows below Tm.
-// At depth 3 this form needs more VGPRs than three waves per SIMD leave and
-// spills (measured 5% slower); depth 3 keeps the per-cell form.
+// cell-by-cell order) and the molten cells and rows are counted for the
+// wavefront from ballots.
 template <int K, int PHYS, int BC, bool EVEN>
 __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <= 3 ? 3 : 2))) void k_step_f64_tb(TbArgs a, StepConst k,
                                                             const LaserTick* __restrict__ plan,
@@ -76,7 +75,8 @@ __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <=
     using Sh = TbShape<K>;
     constexpr bool kLaneOwned = EVEN && BC != LPBF_BC_DIRICHLET;
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
+    // wave-uniform: the band, its rows and the row tests live in SGPRs
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int wg = xcd_remap(blockIdx.x, a.nwg);
     const int task = wg * kTbWaves + wave;
     const int nx = a.nx, ny = a.ny;
@@ -106,6 +106,7 @@ __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <=
     }
     int xcol[2] = {0, 0};     // the lane's two columns (for the index)
     bool lane_owns = false;   // kLaneOwned: the lane's cells count
+    unsigned long long own_mask = 0;   // kLaneOwned: the owning lanes (wave-uniform)
 
     if (task < a.ntasks) {
         const int band = task / a.nstrips, strip = task % a.nstrips;
@@ -115,6 +116,7 @@ __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <=
         const bool own_lane = lane >= Sh::ov && lane < kWave - Sh::ov;
         const bool own0 = own_lane && in0, own1 = own_lane && in1;
         lane_owns = own0;
+        own_mask = __builtin_amdgcn_ballot_w64(own0);
         const int ya = band * a.R;
         const int yb = min(ya + a.R, ny);
         const int y_lo = max(ya - K, 0);
@@ -214,8 +216,12 @@ __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <=
                     best[s] = m;
                     bpos[s] = (res.x == m) ? 2 * r : 2 * r + 1;
                 }
-                if (m > a.Tm) {
-                    molten[s] += (res.x > a.Tm) + (res.y > a.Tm);
+                // molten cells and rows of the owning lanes, counted for the
+                // wavefront from ballots (scalar registers)
+                if (__builtin_amdgcn_ballot_w64(m > a.Tm) & own_mask) {
+                    molten[s] +=
+                        __builtin_popcountll(__builtin_amdgcn_ballot_w64(res.x > a.Tm) & own_mask) +
+                        __builtin_popcountll(__builtin_amdgcn_ballot_w64(res.y > a.Tm) & own_mask);
                     ymin[s] = min(ymin[s], r);
                     ymax[s] = r;
                 }
@@ -269,11 +275,8 @@ __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <=
     if (!a.part) return;
     if (kLaneOwned && !lane_owns) {
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
+        for (int j = 0; j < K; ++j) {   // molten and the row extent are per wavefront
             bpos[j] = -1;
-            ymin[j] = ny;
-            ymax[j] = -1;
-            molten[j] = 0;
             s_dT[j] = s_src[j] = s_loss[j] = 0.0;
         }
     }
@@ -292,7 +295,8 @@ __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <=
         p.ymax = ymax[j];
         const PartialAcc w = wave_reduce(p);
         const double wd = wave_sum(s_dT[j]), ws = wave_sum(s_src[j]), wl = wave_sum(s_loss[j]);
-        const long long wm = wave_sum(static_cast<long long>(molten[j]));
+        const long long wm = kLaneOwned ? static_cast<long long>(molten[j])
+                                        : wave_sum(static_cast<long long>(molten[j]));
         if (lane == 0) {
             s_pa[wave] = w;
             s_sum[0][wave] = wd;
@@ -357,13 +361,11 @@ TbGeom tb_geometry(int nx, int ny, int depth, int rows) {
 template <int K, int PHYS>
 void launch_tb_k(int bc, const TbArgs& a, const StepConst& k, const LaserTick* plan, int step0,
                  hipStream_t s) {
-    // depth 3 keeps the per-cell statistics form (register pressure)
-    constexpr bool kEvenForm = K != 3;
     if (bc == LPBF_BC_DIRICHLET)
         hipLaunchKernelGGL((k_step_f64_tb<K, PHYS, LPBF_BC_DIRICHLET, false>), dim3(a.nwg),
                            dim3(kTbThreads), 0, s, a, k, plan, step0);
     else if (a.nx % 2 == 0)
-        hipLaunchKernelGGL((k_step_f64_tb<K, PHYS, LPBF_BC_MIRROR, kEvenForm>), dim3(a.nwg),
+        hipLaunchKernelGGL((k_step_f64_tb<K, PHYS, LPBF_BC_MIRROR, true>), dim3(a.nwg),
                            dim3(kTbThreads), 0, s, a, k, plan, step0);
     else
         hipLaunchKernelGGL((k_step_f64_tb<K, PHYS, LPBF_BC_MIRROR, false>), dim3(a.nwg),
