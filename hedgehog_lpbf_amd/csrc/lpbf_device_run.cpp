@@ -58,6 +58,11 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
     if (opt_.precision == 32 && (opt_.depth < 1 || opt_.depth > 4))
         throw std::invalid_argument("DeviceRun: depth must be 1..4");
     if (opt_.rows < 0) throw std::invalid_argument("DeviceRun: rows must be >= 0");
+    if (opt_.f64_depth == 0) opt_.f64_depth = 1;
+    if (opt_.precision == 64 && (opt_.f64_depth < 1 || opt_.f64_depth > 4))
+        throw std::invalid_argument("DeviceRun: f64_depth must be 1..4");
+    // the blocked FP64 launches replace the graph batches
+    if (opt_.precision == 64 && opt_.f64_depth > 1) opt_.graph = false;
     T_ref_ = opt_.T_ref < 0.0 ? Tamb_ : opt_.T_ref;
 
     const std::size_t N = static_cast<std::size_t>(nx_) * ny_;
@@ -72,8 +77,22 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
             hip_ok(hipMalloc(&d_b_, N * sizeof(double)), "field");
             nblk_ = lpbf_step_ex_nblocks(nx_, ny_);
             if (nblk_ <= 0) throw std::invalid_argument("DeviceRun: grid too large");
-            hip_ok(hipMalloc(&d_blk_, static_cast<std::size_t>(nblk_) * sizeof(StepStats)),
-                   "block records");
+            std::size_t blk = static_cast<std::size_t>(nblk_) * sizeof(StepStats);
+            if (opt_.f64_depth > 1) {
+                const std::size_t W =
+                    lpbf_f64_blocked_workspace_bytes(nx_, ny_, opt_.f64_depth, opt_.rows);
+                if (W == 0) throw std::invalid_argument("DeviceRun: grid outside the blocked range");
+                blk = std::max(blk, W);
+                std::vector<LaserTick> lt = to_laser(plan_);
+                if (!lt.empty()) {
+                    hip_ok(hipMalloc(reinterpret_cast<void**>(&d_plan_),
+                                     lt.size() * sizeof(LaserTick)), "plan");
+                    hip_ok(hipMemcpyAsync(d_plan_, lt.data(), lt.size() * sizeof(LaserTick),
+                                          hipMemcpyHostToDevice, S(stream_)), "plan upload");
+                    hip_ok(hipStreamSynchronize(S(stream_)), "plan upload");
+                }
+            }
+            hip_ok(hipMalloc(&d_blk_, blk), "block records");
             buf0_ = d_a_;
             if (opt_.graph && opt_.batch % 2 == 0 && !plan_.empty()) {
                 std::vector<LaserTick> lt = to_laser(plan_);
@@ -354,6 +373,37 @@ DeviceRun::Pending DeviceRun::enqueue(int s0, int bn, int slot) {
     if (graph_batch(s0, bn)) {
         for (int i = 0; i < bn; ++i)
             has_col[static_cast<std::size_t>(i)] = cruise(plan_[static_cast<std::size_t>(s0 + i)]);
+    } else if (opt_.precision == 64 && opt_.f64_depth > 1) {
+        int i = 0;
+        while (i < bn) {
+            // a launch covers up to f64_depth steps and ends at a cruise step
+            // (the hot column) or a dump step (the snapshot)
+            int L = 1;
+            while (L < opt_.f64_depth && i + L < bn) {
+                const Tick& t = plan_[static_cast<std::size_t>(s0 + i + L - 1)];
+                if (cruise(t) || (t.dump && snap_cb_)) break;
+                ++L;
+            }
+            lpbf_launch_step_f64_blocked(static_cast<const double*>(d_a_),
+                                         static_cast<double*>(d_b_), &k_, phys_, opt_.bc,
+                                         opt_.T_bc, Tm_, d_plan_, s0 + i, L, opt_.rows,
+                                         d_ring_ + i, d_blk_, s);
+            std::swap(d_a_, d_b_);
+            const int j = i + L - 1;
+            const Tick& tk = plan_[static_cast<std::size_t>(s0 + j)];
+            if (cruise(tk)) {
+                lpbf_launch_extract_col(static_cast<const double*>(d_a_), nx_, ny_,
+                                        reinterpret_cast<const Partial*>(d_ring_ + j),
+                                        d_cols_ + static_cast<std::size_t>(j) * ny_, s);
+                has_col[static_cast<std::size_t>(j)] = 1;
+            }
+            if (tk.dump && snap_cb_) {
+                pend.push_back({tk.step, std::vector<double>(N)});
+                hip_ok(hipMemcpyAsync(pend.back().field.data(), d_a_, N * sizeof(double),
+                                      hipMemcpyDeviceToHost, s), "snapshot");
+            }
+            i += L;
+        }
     } else if (opt_.precision == 64) {
         for (int i = 0; i < bn; ++i) {
             const Tick& tk = plan_[static_cast<std::size_t>(s0 + i)];
@@ -486,6 +536,7 @@ int lpbf_run_create(const StepConst* k, int phys, double Tm, double Lx, double L
             o.T_ref = opt->T_ref;
             o.keep_records = opt->keep_records != 0;
             o.graph = opt->graph != 0;
+            o.f64_depth = opt->f64_depth;
         }
         *out = new lpbf_run(dom, *k, phys, Tm, std::move(ticks), Tamb, o);
     });

@@ -63,6 +63,7 @@ struct CLI {
     int precision = 64, depth = 4, rows = 0;
     int jobs = 1;                  // manifest rows run concurrently (one stream each)
     bool ensemble = false;         // manifest rows stepped together, one launch per step
+    int f64_depth = 1;             // FP64 steps per launch (blocked kernel above 1)
     std::string bc = "mirror";
     double T_bc = 300.0;
 };
@@ -82,6 +83,8 @@ struct CLI {
            "  --batch=K            steps per host synchronisation (default 64)\n"
            "  --precision=64|32    FP64 parity kernels | FP32 temporally blocked path\n"
            "  --depth=K --rows=R   FP32 steps per launch (1..4) and band height\n"
+           "  --f64-depth=K        FP64 steps per launch (1..4); above 1 the temporally\n"
+           "                       blocked kernel between cruise/dump steps (large plates)\n"
            "  --bc=mirror|dirichlet --Tbc=K   boundary condition\n"
            "  --device=N           HIP device (default 0)\n"
            "  --bench              print wall ms + steps/s (no file output)\n"
@@ -131,6 +134,7 @@ CLI parse_cli(int argc, char** argv) {
         else if (key == "--Tbc") c.T_bc = f;
         else if (key == "--jobs") c.jobs = n < 1 ? 1 : n;
         else if (a == "--ensemble") c.ensemble = true;
+        else if (key == "--f64-depth") c.f64_depth = n;
         else if (key == "--check" || key == "--check-rtol" || key == "--check-atol") {
             // the reference binary runs its CPU stepper here (lpbf_cuda.cu:485-514);
             // this binary carries no CPU solver
@@ -177,6 +181,7 @@ RunOptions run_options(CLI const& c) {
     o.batch = c.batch < 1 ? 1 : c.batch;
     o.depth = c.depth;
     o.rows = c.rows;
+    o.f64_depth = c.f64_depth;
     return o;
 }
 

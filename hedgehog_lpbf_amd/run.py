@@ -29,7 +29,7 @@ class DeviceRun:
                  plan: np.ndarray, x_csv=None, wrapped=None, dump=None, phys: int = 0,
                  precision: int = 64, bc: int = 0, T_bc: float = 300.0, batch: int = 64,
                  depth: int = 4, rows: int = 0, T_ref: float = -1.0,
-                 keep_records: bool = True, graph: bool = True):
+                 keep_records: bool = True, graph: bool = True, f64_depth: int = 1):
         plan = np.ascontiguousarray(plan, dtype=np.float64)
         n = int(plan.shape[0]) if plan.size else 0
         self._keep = [plan]
@@ -41,7 +41,7 @@ class DeviceRun:
         du, du_p = _u8(dump, n)
         self._keep += [wr, du]
         opt = RunOptions(precision, bc, T_bc, batch, depth, rows, T_ref, int(keep_records),
-                         int(graph), 0)
+                         int(graph), f64_depth)
         h = C.c_void_p()
         rc = lib().lpbf_run_create(k, phys, Tm, Lx, Ly, Tamb, plan.ctypes.data if n else None,
                                    xc.ctypes.data if xc is not None else None, wr_p, du_p, n,

@@ -18,7 +18,11 @@
 // multiple of `batch` (even) and holds no snapshot is launched as one
 // captured hipGraph: the laser tick and cruise gate of each step come from
 // batch copies in device memory, refreshed before each replay, so the same
-// graph replays for every such batch (kernel nodes only).  Precision 32 runs
+// graph replays for every such batch (kernel nodes only).  With f64_depth > 1
+// (large plates) the FP64 steps run on the temporally blocked FP64 kernel
+// instead, up to f64_depth steps per launch, a launch ending at every cruise
+// or dump step (the hot column and the snapshot need that step's field); the
+// fields are bitwise those of single steps.  Precision 32 runs
 // the temporally blocked FP32 kernel with up to `depth` steps per launch; a launch never
 // crosses a dump step or a batch end, and the hot column (hence W_sub) is
 // taken at the last step of each launch only.
@@ -40,10 +44,12 @@ struct RunOptions {
     double T_bc = 300.0;           // Dirichlet edge temperature
     int batch = 64;                // steps per host synchronisation
     int depth = 4;                 // FP32 steps per launch (1..4)
-    int rows = 0;                  // FP32 band height (0 = built-in)
+    int rows = 0;                  // band height of the blocked kernels (0 = built-in)
     double T_ref = -1.0;           // FP32 theta origin (< 0: Tamb)
     bool keep_records = true;      // keep one StepRecord per step
     bool graph = true;             // FP64: replay full batches as one captured hipGraph
+    int f64_depth = 1;             // FP64 steps per launch (1..4); above 1, the
+                                   // temporally blocked kernel, graph off
 };
 
 struct StepRecord {
@@ -147,7 +153,7 @@ private:
     void* d_a_ = nullptr;          // current field (FP64 T or FP32 theta)
     void* d_b_ = nullptr;
     LaserTick* d_plan_ = nullptr;  // FP32 path: laser per step
-    void* d_blk_ = nullptr;        // FP64 block records / FP32 workspace
+    void* d_blk_ = nullptr;        // FP64 block records / blocked-kernel workspace
     StepStats* d_ring_ = nullptr;  // per-step stats of a batch
     double* d_cols_ = nullptr;     // per-step hot columns of a batch
     double* d_snap_ = nullptr;     // FP32 path: FP64 staging for snapshots

@@ -25,11 +25,13 @@ typedef struct lpbf_run_options {
     double T_bc;          /* Dirichlet edge temperature [K] */
     int    batch;         /* steps per host synchronisation, >= 1 */
     int    depth;         /* FP32: steps per launch, 1..4 */
-    int    rows;          /* FP32: band height, 0 = built-in */
+    int    rows;          /* blocked kernels: band height, 0 = built-in */
     double T_ref;         /* FP32: theta origin [K], < 0 = Tamb */
     int    keep_records;  /* 1: keep one lpbf_step_record per step */
     int    graph;         /* FP64: 1 = replay full batches as a captured hipGraph */
-    int    pad_;
+    int    f64_depth;     /* FP64 steps per launch, 1..4 (0 = 1); above 1 the
+                             temporally blocked kernel, launches ending at cruise
+                             and dump steps, no graph; rows = its band height */
 } lpbf_run_options;
 
 /* Per-step result: the merged statistics of the field after the step, the

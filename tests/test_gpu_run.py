@@ -246,3 +246,50 @@ def test_lpbf_gpu_manifest_and_quiet(tmp_path, ora):
     assert q.returncode == 0, q.stderr
     qrow = [ln for ln in q.stdout.splitlines() if ln.startswith("RESULT,")][0].split(",")
     assert qrow[12:14] == rows[0][12:14]
+
+
+@pytest.mark.parametrize("f64_depth", [2, 4])
+def test_blocked_fp64_run_equals_single_steps(f64_depth):
+    """f64_depth > 1 runs stretches without cruise or dump steps on the
+    temporally blocked FP64 kernel: fields bitwise, per-step records equal up
+    to the last bits of the energy sums (other block partition), summaries
+    equal; both physics, a 64-step batch and an odd one."""
+    mat, pr, dom, dt, n, k, plan, x_csv, wrapped = _case0()
+    n = 500
+    for phys, batch in ((0, 64), (1, 37)):
+        print(f"\n[case] DeviceRun case0 FP64 f64_depth={f64_depth} phys={phys} batch={batch}",
+              flush=True)
+        kk, pl, xc, wr = k, plan[:n], x_csv[:n], wrapped[:n]
+        if phys:
+            dtx = h.dt_cfl_ext(mat, dom, 1.0)
+            kk = h.make_const(mat, pr, dom, dtx)
+            pl, xc, wr = h.single_track_plan(pr.V, dtx, dom.Lx, dom.Ly, n)
+        out = []
+        for fd in (1, f64_depth):
+            with _run(kk, mat, pr, dom, pl, xc, wr, batch=batch, phys=phys, f64_depth=fd) as r:
+                assert r.advance(n) == n
+                out.append((r.records(), r.download(), r.summary()))
+        (ra, Ta, sa), (rb, Tb, sb) = out
+        assert np.array_equal(Ta, Tb)
+        _same_records(ra, rb)
+        assert sa == sb
+        assert sum(1 for x in rb if x.has_col) > 0
+
+
+def test_lpbf_gpu_f64_depth_writes_the_same_outputs(tmp_path):
+    """Snapshots and metrics.csv of a blocked FP64 run equal the single-step
+    run's byte for byte (launches end at every dump and cruise step)."""
+    print("\n[case] lpbf_gpu case0 --f64-depth=3 --snap-every=7 vs --f64-depth=1", flush=True)
+    dirs = []
+    for fd in (1, 3):
+        out = tmp_path / f"d{fd}"
+        r = subprocess.run([EXE, "--case=0", "--tend=3e-4", "--snap-every=7", f"--f64-depth={fd}",
+                            f"--out={out}"], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        dirs.append(out)
+    a, b = dirs
+    assert (a / "metrics.csv").read_bytes() == (b / "metrics.csv").read_bytes()
+    snaps = sorted(p.name for p in a.glob("T_step_*.bin"))
+    assert len(snaps) > 10 and snaps == sorted(p.name for p in b.glob("T_step_*.bin"))
+    for name in snaps:
+        assert (a / name).read_bytes() == (b / name).read_bytes(), name
