@@ -17,8 +17,9 @@
 //   * x neighbours are cross-lane shuffles, y neighbours the register window;
 //     edge conditions are fixed up per row, only in edge strips;
 //   * per level, every owned cell-update feeds the step's statistics (peak T
-//     with its index, molten rows and count, energy sums): FP32 per lane,
-//     FP64 across lanes and workgroups;
+//     with its index, molten rows and count, energy sums): peak and sums per
+//     lane (FP32, the theta sum FP64), molten rows and count per wavefront
+//     from ballots, FP64 across lanes and workgroups;
 //   * four strips form a 256-thread workgroup; workgroup ids are remapped so
 //     consecutive tiles land on the same XCD (shared halo cache lines in L2).
 //
@@ -164,8 +165,9 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
     for (int j = 0; j < K; ++j) acc[j] = LevelAcc{kNegInf, 0x7fffffff, ny, -1, 0, 0.0, 0.f, 0.f};
     double S0 = 0.0;   // sum of the input theta over the lane's cells
     // Lanes 0 and 63 (strip halo) and lanes past column nx own no cells.
-    // Every lane accumulates its own statistics; the ones that own nothing
-    // are dropped once, before the wavefront reductions.
+    // Every lane accumulates its own peak and sums; the ones that own nothing
+    // are dropped once, before the wavefront reductions (the molten counts
+    // are masked with own_mask as they are taken).
     bool owner = false;
     unsigned long long own_mask = 0;   // the owning lanes (wave-uniform)
 
