@@ -59,7 +59,7 @@ class RunOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("bc", C.c_int), ("T_bc", C.c_double),
                 ("batch", C.c_int), ("depth", C.c_int), ("rows", C.c_int),
                 ("T_ref", C.c_double), ("keep_records", C.c_int), ("graph", C.c_int),
-                ("f64_depth", C.c_int)]
+                ("f64_depth", C.c_int), ("cruise_cols", C.c_int), ("pad_", C.c_int)]
 
 
 class StepRecord(C.Structure):

@@ -428,11 +428,14 @@ DeviceRun::Pending DeviceRun::enqueue(int s0, int bn, int slot) {
     } else {
         int i = 0;
         while (i < bn) {
-            // a launch covers up to `depth` steps and ends at a dump step
+            // a launch covers up to `depth` steps and ends at a dump step (and,
+            // with cruise_cols, at a cruise step)
             int L = 1;
-            while (L < opt_.depth && i + L < bn &&
-                   !plan_[static_cast<std::size_t>(s0 + i + L - 1)].dump)
+            while (L < opt_.depth && i + L < bn) {
+                const Tick& t = plan_[static_cast<std::size_t>(s0 + i + L - 1)];
+                if (t.dump || (opt_.cruise_cols && cruise(t))) break;
                 ++L;
+            }
             lpbf_launch_step_f32(static_cast<const float*>(d_a_), static_cast<float*>(d_b_), ld_,
                                  &k_, T_ref_, opt_.bc, opt_.T_bc, Tm_, d_plan_, s0 + i, L,
                                  opt_.rows, d_ring_ + i, d_blk_, s);
@@ -537,6 +540,7 @@ int lpbf_run_create(const StepConst* k, int phys, double Tm, double Lx, double L
             o.keep_records = opt->keep_records != 0;
             o.graph = opt->graph != 0;
             o.f64_depth = opt->f64_depth;
+            o.cruise_cols = opt->cruise_cols != 0;
         }
         *out = new lpbf_run(dom, *k, phys, Tm, std::move(ticks), Tamb, o);
     });

@@ -64,6 +64,7 @@ struct CLI {
     int jobs = 1;                  // manifest rows run concurrently (one stream each)
     bool ensemble = false;         // manifest rows stepped together, one launch per step
     int f64_depth = 1;             // FP64 steps per launch (blocked kernel above 1)
+    bool cruise_cols = false;      // FP32: hot column at every cruise step
     std::string bc = "mirror";
     double T_bc = 300.0;
 };
@@ -85,6 +86,8 @@ struct CLI {
            "  --depth=K --rows=R   FP32 steps per launch (1..4) and band height\n"
            "  --f64-depth=K        FP64 steps per launch (1..4); above 1 the temporally\n"
            "                       blocked kernel between cruise/dump steps (large plates)\n"
+           "  --cruise-cols        FP32: end launches at cruise steps too, so W_sub is\n"
+           "                       sampled at every cruise step as in the FP64 path\n"
            "  --bc=mirror|dirichlet --Tbc=K   boundary condition\n"
            "  --device=N           HIP device (default 0)\n"
            "  --bench              print wall ms + steps/s (no file output)\n"
@@ -135,6 +138,7 @@ CLI parse_cli(int argc, char** argv) {
         else if (key == "--jobs") c.jobs = n < 1 ? 1 : n;
         else if (a == "--ensemble") c.ensemble = true;
         else if (key == "--f64-depth") c.f64_depth = n;
+        else if (a == "--cruise-cols") c.cruise_cols = true;
         else if (key == "--check" || key == "--check-rtol" || key == "--check-atol") {
             // the reference binary runs its CPU stepper here (lpbf_cuda.cu:485-514);
             // this binary carries no CPU solver
@@ -182,6 +186,7 @@ RunOptions run_options(CLI const& c) {
     o.depth = c.depth;
     o.rows = c.rows;
     o.f64_depth = c.f64_depth;
+    o.cruise_cols = c.cruise_cols;
     return o;
 }
 

@@ -29,7 +29,8 @@ class DeviceRun:
                  plan: np.ndarray, x_csv=None, wrapped=None, dump=None, phys: int = 0,
                  precision: int = 64, bc: int = 0, T_bc: float = 300.0, batch: int = 64,
                  depth: int = 4, rows: int = 0, T_ref: float = -1.0,
-                 keep_records: bool = True, graph: bool = True, f64_depth: int = 1):
+                 keep_records: bool = True, graph: bool = True, f64_depth: int = 1,
+                 cruise_cols: bool = False):
         plan = np.ascontiguousarray(plan, dtype=np.float64)
         n = int(plan.shape[0]) if plan.size else 0
         self._keep = [plan]
@@ -41,7 +42,7 @@ class DeviceRun:
         du, du_p = _u8(dump, n)
         self._keep += [wr, du]
         opt = RunOptions(precision, bc, T_bc, batch, depth, rows, T_ref, int(keep_records),
-                         int(graph), f64_depth)
+                         int(graph), f64_depth, int(cruise_cols), 0)
         h = C.c_void_p()
         rc = lib().lpbf_run_create(k, phys, Tm, Lx, Ly, Tamb, plan.ctypes.data if n else None,
                                    xc.ctypes.data if xc is not None else None, wr_p, du_p, n,
@@ -131,7 +132,8 @@ class Ensemble:
         tamb = np.ascontiguousarray([float(r.get("Tamb", 300.0)) for r in runs])
         steps = np.ascontiguousarray(lens, dtype=np.int32)
         self._keep = [ks, plan, xc, wr, tamb, steps]
-        opt = RunOptions(64, bc, T_bc, batch, 4, rows, -1.0, int(keep_records), int(graph), 0)
+        opt = RunOptions(64, bc, T_bc, batch, 4, rows, -1.0, int(keep_records), int(graph), 0, 0,
+                         0)
         h = C.c_void_p()
         rc = lib().lpbf_ensemble_create(
             ks, n, phys, Tm, Lx, Ly, tamb.ctypes.data, plan.ctypes.data if plan.size else None,

@@ -25,7 +25,8 @@
 // fields are bitwise those of single steps.  Precision 32 runs
 // the temporally blocked FP32 kernel with up to `depth` steps per launch; a launch never
 // crosses a dump step or a batch end, and the hot column (hence W_sub) is
-// taken at the last step of each launch only.
+// taken at the last step of each launch only, or, with cruise_cols, at every
+// cruise step (launches also end there).
 #pragma once
 
 #include <cstdint>
@@ -50,6 +51,8 @@ struct RunOptions {
     bool graph = true;             // FP64: replay full batches as one captured hipGraph
     int f64_depth = 1;             // FP64 steps per launch (1..4); above 1, the
                                    // temporally blocked kernel, graph off
+    bool cruise_cols = false;      // FP32: launches also end at cruise steps, so
+                                   // every cruise step gets its hot column
 };
 
 struct StepRecord {

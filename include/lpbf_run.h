@@ -32,6 +32,10 @@ typedef struct lpbf_run_options {
     int    f64_depth;     /* FP64 steps per launch, 1..4 (0 = 1); above 1 the
                              temporally blocked kernel, launches ending at cruise
                              and dump steps, no graph; rows = its band height */
+    int    cruise_cols;   /* FP32: 1 = launches also end at every cruise step, so
+                             each cruise step gets its hot column (W_sub) as in
+                             the FP64 path; 0 = only at launch ends */
+    int    pad_;
 } lpbf_run_options;
 
 /* Per-step result: the merged statistics of the field after the step, the
@@ -112,8 +116,10 @@ int  lpbf_ensemble_records(const lpbf_ensemble* e, int m, int first, int n,
 
 #ifdef __cplusplus
 }
+static_assert(sizeof(lpbf_run_options) == 64, "lpbf_run_options layout");
 static_assert(sizeof(lpbf_step_record) == 80, "lpbf_step_record layout");
 #else
+_Static_assert(sizeof(lpbf_run_options) == 64, "lpbf_run_options layout");
 _Static_assert(sizeof(lpbf_step_record) == 80, "lpbf_step_record layout");
 #endif
 

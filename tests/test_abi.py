@@ -24,6 +24,10 @@ def _declared():
     return names
 
 
+def test_run_options_layout():
+    assert C.sizeof(h.RunOptions) == 64 and C.sizeof(h.StepRecord) == 80
+
+
 def test_library_exports_every_declared_symbol():
     lib = h.lib()
     names = _declared()
@@ -82,7 +86,7 @@ def test_ensemble_create_rejects_bad_runs_before_touching_a_device():
     h.lib().lpbf_get_last_error()
     # precision other than 64 is refused by the C entry point itself
     lib, k = h.lib(), (h.StepConst * 1)(member()["k"])
-    opt = h.RunOptions(32, 0, 300.0, 64, 4, 0, -1.0, 0, 1, 0)
+    opt = h.RunOptions(32, 0, 300.0, 64, 4, 0, -1.0, 0, 1, 0, 0, 0)
     n = (C.c_int * 1)(4)
     plan = np.zeros((4, 3))
     out = C.c_void_p()

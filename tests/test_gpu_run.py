@@ -293,3 +293,27 @@ def test_lpbf_gpu_f64_depth_writes_the_same_outputs(tmp_path):
     assert len(snaps) > 10 and snaps == sorted(p.name for p in b.glob("T_step_*.bin"))
     for name in snaps:
         assert (a / name).read_bytes() == (b / name).read_bytes(), name
+
+
+def test_fp32_cruise_cols_sample_every_cruise_step(ora):
+    """cruise_cols ends FP32 launches at cruise steps too: every cruise step
+    gets its hot column, as in the FP64 path, and the steady sub-grid width
+    is within 0.5 um of the FP64 run's (the FP32 field is within 0.5 K; the
+    Tm crossing moves by dT / |dT/dy|, far below a cell at the melt edge)."""
+    print("\n[case] DeviceRun case0 FP32 depth 4 cruise_cols vs FP64", flush=True)
+    mat, pr, dom, dt, n, k, plan, x_csv, wrapped = _case0()
+    cruise = (~wrapped.astype(bool)) & (x_csv >= 0.45 * dom.Lx) & (x_csv <= 0.85 * dom.Lx)
+    with _run(k, mat, pr, dom, plan, x_csv, wrapped, precision=32, depth=4,
+              cruise_cols=True) as r32:
+        r32.advance(n)
+        rec32, s32 = r32.records(), r32.summary()
+    with _run(k, mat, pr, dom, plan, x_csv, wrapped) as r64:
+        r64.advance(n)
+        rec64, s64 = r64.records(), r64.summary()
+    has = np.array([bool(x.has_col) for x in rec32])
+    assert cruise.sum() > 100 and np.array_equal(has, cruise)
+    w32 = np.array([x.W_sub_um for x in rec32])[cruise]
+    w64 = np.array([x.W_sub_um for x in rec64])[cruise]
+    assert np.abs(w32 - w64).max() <= 0.5
+    assert abs(s32[2] - s64[2]) <= 0.5 and s64[2] > 10.0
+    assert abs(s32[0] - s64[0]) <= 0.5
