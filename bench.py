@@ -55,7 +55,8 @@ def plate_setup(nx=8192, ny=8192):
 
 
 def cpu_threads():
-    """Host threads for the CPU baseline: the box's CPU share, at most 16."""
+    """Host threads for the CPU baseline: the process's CPU affinity, at most 16
+    (the host share of one GPU)."""
     n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     return max(1, min(16, n or 1))
 
@@ -478,8 +479,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every box here "
-              "has one GPU; the multi-GPU sharding is a design note only)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build "
+              "runs on one GPU; the multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
