@@ -479,7 +479,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build "
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the benchmark "
               "runs on one GPU; the multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
