@@ -520,9 +520,12 @@ bool valid_grid(const StepConst* k) { return k && shape_ok(k->nx, k->ny); }
 // the chip; a launch that is latency-bound gains more from parallel rows
 // than it loses to the extra halo rows.
 int plan_rows(int nx, int ny) {
+    // shorter bands spread a small plate over more CUs and shorten each
+    // wavefront's chain; down to one row (case0: 5.5 us per step at one
+    // row against 6.3 at two)
     const int nwx = (nx + kBlockCols - 1) / kBlockCols;
     int R = kRowsPerWave;
-    while (R > 2 && static_cast<long long>(nwx) * ((ny + R - 1) / R) < 4096) R /= 2;
+    while (R > 1 && static_cast<long long>(nwx) * ((ny + R - 1) / R) < 4096) R /= 2;
     return R;
 }
 
@@ -702,12 +705,20 @@ void lpbf_launch_step_ex_batch_flush(const double* T, int nx, int ny, int j,
 
 int lpbf_step_ensemble_rows(int nx, int ny, int n_runs) {
     if (!shape_ok(nx, ny) || n_runs < 1) return 0;
-    // measured on 240x120 rows (64 and 256 of them): short bands pay the
-    // per-wavefront set-up and reductions, few wavefronts leave SIMDs idle;
-    // 4-row bands up to ~8k wavefronts in all, longer ones beyond
+    // measured on 240x120 rows (1, 64 and 256 of them): short bands pay the
+    // per-wavefront set-up and reductions, few wavefronts leave SIMDs idle.
+    // Few rows (under ~1k wavefronts at 4-row bands): the shortest bands
+    // that keep ~4k wavefronts; otherwise 4-row bands up to ~8k wavefronts
+    // in all, longer ones beyond
     const long long nwx = (nx + kBlockCols - 1) / kBlockCols;
+    auto waves = [&](int R) { return nwx * ((ny + R - 1) / R) * n_runs; };
     int R = 4;
-    while (R < 32 && nwx * ((ny + R - 1) / R) * n_runs > 8192) R *= 2;
+    if (waves(4) < 1024) {
+        R = 1;
+        while (R < 4 && waves(R) > 4096) R *= 2;
+        return R;
+    }
+    while (R < 32 && waves(R) > 8192) R *= 2;
     return R;
 }
 

@@ -4,8 +4,9 @@
 // GpuStepTask + MetricsTask pair on the same Domain) as one FP64 launch per
 // step for all rows instead of one small launch per row and step.
 //
-// The default plate (240 x 120) gives a step four or five workgroups: one
-// run leaves nearly every CU idle and is bound by launch latency.  Here
+// The default plate (240 x 120) is 28,800 cells: one run's step is a few
+// dozen small workgroups, leaves most CUs idle and is bound by launch
+// latency.  Here
 // every run keeps its own fields, constants, laser plan and cruise flags in
 // HBM, and batch step j of the ensemble is one lpbf_launch_step_ensemble
 // over all runs (per run: the step's compute workgroups plus one that
