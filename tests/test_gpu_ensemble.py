@@ -52,23 +52,24 @@ def _sums(r):
     return np.array([r.stats.sum_dT, r.stats.sum_src, r.stats.sum_loss])
 
 
-@pytest.mark.parametrize("phys", [0, 1])
-def test_ensemble_equals_device_runs(phys):
+@pytest.mark.parametrize("phys,bc", [(0, 0), (1, 0), (0, 1)])
+def test_ensemble_equals_device_runs(phys, bc):
     t_end = 6e-4
-    print(f"\n[case] Ensemble of {len(ROWS)} runs phys={phys} t_end={t_end} vs DeviceRuns",
-          flush=True)
+    print(f"\n[case] Ensemble of {len(ROWS)} runs phys={phys} bc={bc} t_end={t_end} vs "
+          "DeviceRuns", flush=True)
     built = [_member(r, phys, t_end) for r in ROWS]
     runs = [b[0] for b in built]
     mat, dom = built[0][1], built[0][2]
     lens = [len(m["plan"]) for m in runs]
     if phys:
         assert len(set(lens)) > 1          # kmult changes dt: runs of different length
-    with h.Ensemble(runs, Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, phys=phys, batch=64) as ens:
+    with h.Ensemble(runs, Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, phys=phys, batch=64, bc=bc,
+                    T_bc=300.0) as ens:
         assert ens.advance(10 ** 6) == max(lens)
         assert ens.graph_launches() >= max(lens) // 64
         got = [(ens.download(i), ens.records(i), ens.summary(i)) for i in range(len(runs))]
     for i, m in enumerate(runs):
-        with _device_run(m, mat, dom, phys, batch=64) as dr:
+        with _device_run(m, mat, dom, phys, batch=64, bc=bc, T_bc=300.0) as dr:
             assert dr.advance(lens[i]) == lens[i]
             T, recs, summ = dr.download(), dr.records(), dr.summary()
         Te, re, se = got[i]

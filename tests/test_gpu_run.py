@@ -256,9 +256,9 @@ def test_blocked_fp64_run_equals_single_steps(f64_depth):
     equal; both physics, a 64-step batch and an odd one."""
     mat, pr, dom, dt, n, k, plan, x_csv, wrapped = _case0()
     n = 500
-    for phys, batch in ((0, 64), (1, 37)):
-        print(f"\n[case] DeviceRun case0 FP64 f64_depth={f64_depth} phys={phys} batch={batch}",
-              flush=True)
+    for phys, batch, bc in ((0, 64, 0), (1, 37, 0), (0, 64, 1)):
+        print(f"\n[case] DeviceRun case0 FP64 f64_depth={f64_depth} phys={phys} batch={batch} "
+              f"bc={bc}", flush=True)
         kk, pl, xc, wr = k, plan[:n], x_csv[:n], wrapped[:n]
         if phys:
             dtx = h.dt_cfl_ext(mat, dom, 1.0)
@@ -266,7 +266,8 @@ def test_blocked_fp64_run_equals_single_steps(f64_depth):
             pl, xc, wr = h.single_track_plan(pr.V, dtx, dom.Lx, dom.Ly, n)
         out = []
         for fd in (1, f64_depth):
-            with _run(kk, mat, pr, dom, pl, xc, wr, batch=batch, phys=phys, f64_depth=fd) as r:
+            with _run(kk, mat, pr, dom, pl, xc, wr, batch=batch, phys=phys, f64_depth=fd,
+                      bc=bc, T_bc=300.0) as r:
                 assert r.advance(n) == n
                 out.append((r.records(), r.download(), r.summary()))
         (ra, Ta, sa), (rb, Tb, sb) = out
