@@ -14,8 +14,8 @@ autotuned);
 (one bench step = one time step of every instance); `case0` times the
 reference's published case on the device-resident FP64 path; `ensemble`
 steps a 64-row manifest sweep of that plate together (lpbf::Ensemble) and
-times the row-by-row loop beside it; `run8k_f64` runs the device-resident
-FP64 run (per-step metrics) on the 8192 x 8192 plate.
+times the row-by-row loop beside it; `run8k_f64` / `run8k_f32` run the
+device-resident run (per-step metrics) on the 8192 x 8192 plate.
 
 Prints one JSON line: value = cell-updates/s over the timed steps, with the
 inputs already resident in HBM; `roofline` for the dominant kernel (step
@@ -229,23 +229,26 @@ def batch_bench(args):
     b.close()
 
 
-def run8k_bench(args):
+def run8k_bench(args, precision):
     """The device-resident run (lpbf::DeviceRun, the reference GpuStepTask +
-    MetricsTask pair) on the 8192^2 FP64 plate: per-step statistics rings,
+    MetricsTask pair) on the 8192^2 plate: per-step statistics rings,
     per-step metrics finalised on the host, one synchronisation per 64-step
-    batch, the steps on the temporally blocked FP64 kernel (--depth, default
-    2; 1 = single-step kernels).  The raster plan has no cruise gate, so no
-    hot columns are taken."""
+    batch.  FP64: the steps on the temporally blocked FP64 kernel (--depth,
+    default 2; 1 = single-step kernels); FP32: the FP32 kernel (--depth,
+    default 4).  The raster plan has no cruise gate, so no hot columns are
+    taken."""
     import torch
 
     import hedgehog_lpbf_amd as h
     mat, pr, dom, dt, k, plan = plate_setup(args.n, args.n)
-    depth = args.depth if args.depth > 0 else 2
+    depth = args.depth if args.depth > 0 else (2 if precision == 64 else 4)
+    rows = args.rows if args.rows > 0 or precision == 64 else 46
     warm = args.warmup if args.warmup is not None else 64
     steps = args.steps if args.steps is not None else 1024
+    kw = dict(f64_depth=depth) if precision == 64 else dict(precision=32, depth=depth)
     with h.DeviceRun(k, Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, Tamb=pr.Tamb,
-                     plan=plan[:warm + steps], batch=64, keep_records=False,
-                     f64_depth=depth, rows=args.rows) as r:
+                     plan=plan[:warm + steps], batch=64, keep_records=False, rows=rows,
+                     **kw) as r:
         r.noise(0, pr.Tamb, 0.5)
         r.advance(warm)
         torch.cuda.synchronize()
@@ -257,12 +260,14 @@ def run8k_bench(args):
     print(json.dumps({
         "metric": "cell-updates/s", "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": steps, "warmup": warm, "ms_per_step": wall * 1e3 / steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if precision == 64 else "f32", "data": "synthetic",
         "config": {"workload": f"{dom.nx}x{dom.ny} plate, 20-track raster P 250 W V 1.2 m/s, "
-                               "FP64 device-resident run (per-step statistics and metrics, "
-                               f"batch 64), blocked FP64 kernel depth {depth}",
-                   "f64_depth": depth, "band_rows": args.rows, "peak_T_K": peak_T},
+                               f"FP{precision} device-resident run (per-step statistics and "
+                               f"metrics, batch 64), "
+                               + (f"blocked FP64 kernel depth {depth}" if precision == 64
+                                  else f"FP32 kernel depth {depth}"),
+                   "depth": depth, "band_rows": rows, "peak_T_K": peak_T},
         "roofline": None, "cpu_baseline": None}), flush=True)
 
 
@@ -501,11 +506,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="plate8k_f32",
                     choices=["plate8k_f32", "plate8k_f64", "batch256", "case0", "ensemble",
-                             "run8k_f64", "surrogate", "copy"],
+                             "run8k_f64", "run8k_f32", "surrogate", "copy"],
                     help="batch256: 256 uneven instances in one hipGraph launch; case0: the "
                          "reference's published case on the device-resident path; ensemble: "
-                         "--runs manifest rows of that plate stepped together; run8k_f64: "
-                         "the device-resident FP64 run on the 8192^2 plate; copy: a "
+                         "--runs manifest rows of that plate stepped together; run8k_f64 / "
+                         "run8k_f32: the device-resident run on the 8192^2 plate; copy: a "
                          "plain device copy of one n x n FP32 plate per step, the reference "
                          "for calibrating HBM byte counters")
     ap.add_argument("--depth", type=int, default=0,
@@ -532,8 +537,8 @@ def main():
         return batch_bench(args)
     if args.workload == "case0":
         return case0_bench(args)
-    if args.workload == "run8k_f64":
-        return run8k_bench(args)
+    if args.workload in ("run8k_f64", "run8k_f32"):
+        return run8k_bench(args, 64 if args.workload == "run8k_f64" else 32)
     if args.workload == "ensemble":
         return ensemble_bench(args)
     if args.workload == "surrogate":

@@ -23,5 +23,5 @@ def test_every_workload_is_listed():
     r = _bench("--help")
     assert r.returncode == 0
     for w in ("plate8k_f32", "plate8k_f64", "batch256", "case0", "ensemble", "run8k_f64",
-              "surrogate", "copy"):
+              "run8k_f32", "surrogate", "copy"):
         assert w in r.stdout, w
