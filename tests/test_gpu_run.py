@@ -318,3 +318,43 @@ def test_fp32_cruise_cols_sample_every_cruise_step(ora):
     assert np.abs(w32 - w64).max() <= 0.5
     assert abs(s32[2] - s64[2]) <= 0.5 and s64[2] > 10.0
     assert abs(s32[0] - s64[0]) <= 0.5
+
+
+def test_fp32_full_plate_50k_steps_within_half_kelvin():
+    """The FP32 accuracy target at its full size: the 8192^2 plate (dx 5 um,
+    dt 1.2 us, 20-track raster, P 250 W, V 1.2 m/s), 50,000 steps from the
+    same noisy start (300 K +/- 0.5 K), the FP32 kernel (depth 4, 46-row
+    bands) against the blocked FP64 kernel (depth 2), which equals the
+    oracle bit for bit wherever the oracle can run
+    (tests/test_gpu_f64_blocked.py).  Field, every step's peak temperature
+    and the run's peak within 0.5 K; the peak melt width, counted in whole
+    rows, within one row."""
+    print("\n[case] DeviceRun 8192^2 raster 50k steps FP32 depth 4 vs FP64 depth 2",
+          flush=True)
+    n, dx, dt = 8192, 5e-6, 1.2e-6
+    mat = h.Material(k=7.0, rho=4430.0, cp=526.0, Tm=1609.0, eps=0.40, eta=0.35)
+    pr = h.Process(P=250.0, V=1.2, r0=50e-6, h=50e-6, Tamb=300.0, Tsub=300.0)
+    dom = h.Domain(Lx=(n - 1) * dx, Ly=(n - 1) * dx, nx=n, ny=n)
+    steps = 50000
+    r = h.centred_raster(dom.Lx, dom.Ly, pr.V, dt, steps, 20, 80e-6)
+    plan = h.raster_plan(r["x0"], r["y0"], r["length"], r["hatch"], r["n_tracks"], r["V"],
+                         dt, steps)
+    k = h.make_const(mat, pr, dom, dt)
+    out = {}
+    for name, kw in (("f64", dict(f64_depth=2)), ("f32", dict(precision=32, depth=4, rows=46))):
+        with h.DeviceRun(k, Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, Tamb=pr.Tamb, plan=plan,
+                         **kw) as run:
+            run.noise(0, pr.Tamb, 0.5)
+            assert run.advance(steps) == steps
+            tmax = np.array([x.stats.tmax for x in run.records()])
+            out[name] = (run.download(), tmax, run.summary())
+        print(f"[case] {name} done: peak T {out[name][2][0]:.3f} K", flush=True)
+    (T64, t64, s64), (T32, t32, s32) = out["f64"], out["f32"]
+    err = np.abs(T32 - T64).max()
+    print(f"[case] max |T32 - T64| = {err:.3e} K over {steps} steps", flush=True)
+    assert len(t64) == len(t32) == steps
+    assert err <= 0.5
+    assert np.abs(t32 - t64).max() <= 0.5
+    assert abs(s32[0] - s64[0]) <= 0.5 and s64[0] > mat.Tm
+    assert abs(s32[1] - s64[1]) <= dx * 1e6 + 1e-9
+    assert T64.max() > 1000.0          # the last track is still hot at the end
