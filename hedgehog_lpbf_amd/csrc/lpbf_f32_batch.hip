@@ -125,6 +125,7 @@ struct Batch {
     bool groups_valid = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<int> graph_parity;   // each group's parity when the graph was captured
 
     ~Batch() {
         drop_graph();
@@ -139,6 +140,7 @@ struct Batch {
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
+        graph_parity.clear();
     }
 
     static void free_group(Group& g) {
@@ -395,11 +397,19 @@ int lpbf_f32_batch_capture(lpbf_f32_batch* b, int step0, int n_steps, hipStream_
         restore();   // and an even launch count leaves every field where it started
         lpbf_amd::ok(hipStreamEndCapture(stream, &B.graph));
         lpbf_amd::ok(hipGraphInstantiateWithFlags(&B.exec, B.graph, 0));
+        B.graph_parity = p0;
     });
 }
 
 int lpbf_f32_batch_launch_graph(lpbf_f32_batch* b, hipStream_t stream) {
     if (!b || !b->b.exec) return fail(LPBF_ERR_INVALID_ARGUMENT);
+    // the captured launches read and write fixed buffers: an eager enqueue
+    // with an odd launch count since the capture has moved some field to the
+    // other buffer, and a replay would then step a stale field
+    Batch const& B = b->b;
+    if (B.graph_parity.size() != B.groups.size()) return fail(LPBF_ERR_INVALID_ARGUMENT);
+    for (size_t q = 0; q < B.groups.size(); ++q)
+        if (B.groups[q].p != B.graph_parity[q]) return fail(LPBF_ERR_INVALID_ARGUMENT);
     return guarded([&] { lpbf_amd::ok(hipGraphLaunch(b->b.exec, stream)); });
 }
 

@@ -220,6 +220,15 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                 set_elem(L, laste, 0.f);
             }
         };
+        // Dirichlet edge cells are held, not updated, and stay out of the
+        // energy sums: columns 0 and nx-1 drop out of a row's theta sum
+        auto dir_cols = [&](float4 v) {
+            if (BC == LPBF_BC_DIRICHLET && edge) {
+                if (has_left && lane == 1) v.x = 0.f;
+                if (has_last && lane == lastl) set_elem(v, laste, 0.f);
+            }
+            return v;
+        };
         // A row's contribution to the lane's sums; on the lane holding column
         // nx-1, columns >= nx drop out.
         auto row_sum = [&](float4 v) {
@@ -332,14 +341,17 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                     out4[e] += sv;
                     // lanes past column nx own nothing and are dropped later;
                     // only a straddling lane needs the per-column test
-                    ssum += (!RAGGED || col4 + e < nx) ? sv : 0.f;
+                    const int cc = col4 + e;
+                    if (BC == LPBF_BC_DIRICHLET) ssum += (cc >= 1 && cc < nx - 1) ? sv : 0.f;
+                    else ssum += (!RAGGED || cc < nx) ? sv : 0.f;
                 }
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) out4[e] = fmaf(a.cx, xm[e] + xp[e], out4[e]);
             float4 res = make_float4(out4[0], out4[1], out4[2], out4[3]);
             float4 L4 = make_float4(loss4[0], loss4[1], loss4[2], loss4[3]);
-            if (GEN && BC == LPBF_BC_DIRICHLET && (r == 0 || r == ny - 1)) {
+            const bool drow = GEN && BC == LPBF_BC_DIRICHLET && (r == 0 || r == ny - 1);
+            if (drow) {
                 res = make_float4(a.theta_bc, a.theta_bc, a.theta_bc, a.theta_bc);
                 L4 = make_float4(0.f, 0.f, 0.f, 0.f);
                 ssum = 0.f;
@@ -377,7 +389,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                     A.ymin = min(A.ymin, r);
                     A.ymax = r;
                 }
-                A.S += static_cast<double>(row_sum(res));
+                A.S += drow ? 0.0 : static_cast<double>(row_sum(dir_cols(res)));
                 A.SL += row_sum(L4);
                 A.SS += ssum;   // 0 off the source rows: an add, no select
                 // keep each running sum updated here: left alone the compiler
@@ -402,7 +414,8 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             if (!GEN || i < y_hi) {
                 if (edge && BC == LPBF_BC_MIRROR) fix_mirror(v);
                 if (i >= ya && i < yb) {
-                    S0 += static_cast<double>(row_sum(v));
+                    const bool drow = GEN && BC == LPBF_BC_DIRICHLET && (i == 0 || i == ny - 1);
+                    S0 += drow ? 0.0 : static_cast<double>(row_sum(dir_cols(v)));
                     pin(S0);
                 }
             } else if (i == ny) {

@@ -397,8 +397,16 @@ extern "C" {
 
 size_t lpbf_f64_blocked_workspace_bytes(int nx, int ny, int depth, int rows) {
     if (!tb_shape_ok(nx, ny) || depth < 1 || depth > kTbMaxDepth || rows < 0) return 0;
-    const TbGeom g = tb_geometry(nx, ny, depth, rows);
-    return sizeof(StepStats) * static_cast<size_t>(depth) * g.nwg;
+    // every launch of depth d <= depth writes d * nwg(d) records, and with the
+    // built-in band height (rows == 0) a shorter depth can mean more
+    // workgroups, since the minimum band height grows with the depth
+    size_t need = 0;
+    for (int d = 1; d <= depth; ++d) {
+        const TbGeom g = tb_geometry(nx, ny, d, rows);
+        const size_t b = sizeof(StepStats) * static_cast<size_t>(d) * static_cast<size_t>(g.nwg);
+        if (b > need) need = b;
+    }
+    return need;
 }
 
 void lpbf_launch_step_f64_blocked(const double* Tin, double* Tout, const StepConst* k, int phys,

@@ -82,7 +82,10 @@ int  lpbf_f32_batch_enqueue(lpbf_f32_batch* b, int step0, int n_steps, hipStream
  * with an odd count is made even by splitting its last multi-step launch,
  * and an odd count of single-step launches is refused. */
 int  lpbf_f32_batch_capture(lpbf_f32_batch* b, int step0, int n_steps, hipStream_t stream);
-/* Launches the captured graph on `stream` (may be repeated). */
+/* Launches the captured graph on `stream` (may be repeated).  Refused with
+ * LPBF_ERR_INVALID_ARGUMENT when an eager lpbf_f32_batch_enqueue since the
+ * capture has left some field in the other buffer than at capture (an odd
+ * number of launches); an even number of eager launches restores it. */
 int  lpbf_f32_batch_launch_graph(lpbf_f32_batch* b, hipStream_t stream);
 
 /* 0 if theta0, 1 if theta1 holds instance i's latest field; < 0 on error. */

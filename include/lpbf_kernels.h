@@ -183,8 +183,9 @@ void lpbf_launch_stats_finalize(const StepStats* parts, int nblocks, int nx,
  * rows: band height (0 = built-in).  If stats != NULL it receives `depth`
  * StepStats (one per step, merged on the device; the FP64 sums are added in
  * a different order than lpbf_launch_step_ex's, so they agree to rounding)
- * and work must hold lpbf_f64_blocked_workspace_bytes(nx, ny, depth, rows)
- * bytes of device memory. */
+ * and work must hold lpbf_f64_blocked_workspace_bytes(nx, ny, D, rows)
+ * bytes of device memory, which covers every launch of depth <= D with the
+ * same rows. */
 size_t lpbf_f64_blocked_workspace_bytes(int nx, int ny, int depth, int rows);
 void lpbf_launch_step_f64_blocked(const double* Tin, double* Tout, const StepConst* k,
                                   int phys, int bc, double T_bc, double Tm,

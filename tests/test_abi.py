@@ -144,3 +144,37 @@ def test_plans_equal_oracle(ora):
     want = ora.raster_plan(ora.OraRaster(**r), 1.2e-6, 50000)
     assert np.array_equal(got, want)
     assert got[:, 2].sum() > 0.9 * 50000 and got[-1, 2] in (0.0, 1.0)
+
+
+def _tb_records(nx, ny, d, rows):
+    """Block records one blocked FP64 launch of depth d writes: d per
+    workgroup of four strip-bands (strips of 128 loaded columns overlapping by
+    2*ceil(d/2) per side; built-in band height: about four rounds of 3 * 1024
+    wavefronts, at least 8*d rows)."""
+    out = 128 - 4 * ((d + 1) // 2)
+    nstrips = -(-nx // out)
+    if rows == 0:
+        bands = -(-(4 * 3 * 1024) // nstrips)
+        rows = min(max(-(-ny // bands), 8 * d), ny)
+    tasks = nstrips * -(-ny // rows)
+    return d * -(-tasks // 4)
+
+
+@pytest.mark.parametrize("nx,ny", [(240, 120), (512, 512), (2048, 1024), (8192, 8192),
+                                   (1001, 77), (130, 3000), (64, 9)])
+@pytest.mark.parametrize("rows", [0, 16, 33])
+def test_f64_blocked_workspace_covers_shorter_launches(nx, ny, rows):
+    """A device-resident run sizes the blocked kernel's workspace for its
+    deepest launch and then also issues shorter ones (at cruise and dump
+    steps): the size for depth D must hold the records of every depth d <= D.
+    With the built-in band height a shorter depth can need more records."""
+    lib = h.lib()
+    sz = C.sizeof(h.StepStats)
+    for D in range(1, 5):
+        ws = lib.lpbf_f64_blocked_workspace_bytes(nx, ny, D, rows)
+        for d in range(1, D + 1):
+            assert ws >= sz * _tb_records(nx, ny, d, rows), (D, d)
+    assert lib.lpbf_f64_blocked_workspace_bytes(nx, ny, 1, rows) == sz * _tb_records(nx, ny, 1, rows)
+    # the cases where depth-D sizing alone was short
+    if (nx, ny, rows) == (240, 120, 0):
+        assert _tb_records(240, 120, 3, 0) > _tb_records(240, 120, 4, 0)

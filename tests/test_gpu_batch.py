@@ -93,18 +93,37 @@ def test_graph_replays_equal_eager(ora):
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         ga.capture(0, n, s.cuda_stream)
+    def same(tag):
+        for i in range(len(specs)):
+            assert torch.equal(ga.theta(i)[:, :specs[i].k.nx], eb.theta(i)[:, :specs[i].k.nx]), \
+                (tag, i)
+
     for rep in range(3):
         with torch.cuda.stream(s):
             ga.launch_graph(s.cuda_stream)
         eb.enqueue(0, n)
         torch.cuda.synchronize()
         h.check()
+        same(rep)
         for i in range(len(specs)):
             assert ga.launches(n) == eb.launches(n)
-            assert torch.equal(ga.theta(i)[:, :specs[i].k.nx], eb.theta(i)[:, :specs[i].k.nx]), \
-                (rep, i)
             sa, sb = ga.stats(i, 0, n), eb.stats(i, 0, n)
             assert [bytes(x) for x in sa] == [bytes(x) for x in sb], (rep, i)
+        # eager work on the captured batch between replays: one single step
+        # (an odd launch count) moves every field to its other buffer, so a
+        # replay is refused until a second step moves them back
+        with torch.cuda.stream(s):
+            ga.enqueue(n + 2 * rep, 1, s.cuda_stream)
+        eb.enqueue(n + 2 * rep, 1)
+        with pytest.raises(h.LpbfError):
+            with torch.cuda.stream(s):
+                ga.launch_graph(s.cuda_stream)
+        with torch.cuda.stream(s):
+            ga.enqueue(n + 2 * rep + 1, 1, s.cuda_stream)
+        eb.enqueue(n + 2 * rep + 1, 1)
+        torch.cuda.synchronize()
+        h.check()
+        same((rep, "eager"))
     ga.close()
     eb.close()
 
@@ -113,8 +132,9 @@ def test_autotune_keeps_fields_and_picks_valid_tiling(ora):
     print("\n[case] f32 batch autotune", flush=True)
     specs = _specs(ora, 16)
     b = h.F32Batch(specs)
-    b.enqueue(0, 3)                          # odd history: some fields now in theta1
+    b.enqueue(0, 1)                          # one launch: every field now in theta1
     torch.cuda.synchronize()
+    assert all(h.lib().lpbf_f32_batch_current(b._h, i) == 1 for i in range(len(specs)))
     before = [b.field(i) for i in range(len(specs))]
     ms = b.autotune(4)
     assert ms > 0
@@ -123,7 +143,7 @@ def test_autotune_keeps_fields_and_picks_valid_tiling(ora):
         assert 2 <= d <= 4 and (r == 0 or r >= 2 * d) and m == 3
     for i in range(len(specs)):
         assert np.array_equal(b.field(i), before[i]), i
-    b.enqueue(3, 5)
+    b.enqueue(1, 5)
     torch.cuda.synchronize()
     h.check()
     b.close()

@@ -194,6 +194,34 @@ def test_dirichlet_run_against_oracle(ora):
     assert err <= 0.05
 
 
+@pytest.mark.parametrize("x_cells,y_frac", [(1.5, 0.5), (-1.0, 0.02)])
+def test_dirichlet_stats_against_oracle(ora, x_cells, y_frac):
+    """Dirichlet edges, beam parked at the left edge (and at a corner): the
+    held edge cells take no part in the energy sums (ora_step_ex), so the
+    FP32 sums of dT, source and loss follow the FP64 oracle's at every step,
+    the first included, although the noisy start is reset to T_bc on the
+    edges by that step.  The ragged width exercises the last lane."""
+    print(f"\n[case] f32 dirichlet stats beam at x={x_cells} cells y={y_frac}", flush=True)
+    nx, ny, n = 203, 97, 16
+    s = _setup(ora, nx, ny)
+    s.proc.Tamb = s.proc.Tsub = 300.0
+    k = ora.make_const(s, ora.dt_cfl(s))
+    plan = np.zeros((n, 3))
+    plan[:, 0] = (x_cells + 0.1 * np.arange(n)) * k.dx
+    plan[:, 1] = y_frac * s.dom.Ly
+    plan[:, 2] = 1.0
+    T0 = 300.0 + 40.0 * ora.noise_field(5, 0.0, 0.5, (ny, nx))
+    Tref, rst = ora.run_plan(k, plan, T0, 0, 1, 300.0)
+    th, sts, _ = run_f32(k, plan, T0, n, 4, bc=1)
+    assert np.abs(300.0 + th.astype(np.float64) - Tref).max() <= 0.05
+    for i in range(n):
+        x, y = sts[i], rst[i]
+        assert x.sum_src == pytest.approx(y.sum_src, rel=1e-4), i
+        assert x.sum_loss == pytest.approx(y.sum_loss, rel=1e-3, abs=1e-3), i
+        # FP32 theta rounding: a few 1e-5 K per cell at most
+        assert abs(x.sum_dT - y.sum_dT) <= 2e-4 * abs(y.sum_dT) + 3e-5 * nx * ny, i
+
+
 def test_noise_f32_equals_oracle(ora):
     print("\n[case] f32 noise", flush=True)
     nx, ny, ld = 123, 45, 128
