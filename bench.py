@@ -62,50 +62,65 @@ def cpu_threads():
     return max(1, min(16, n or 1))
 
 
-def cpu_baseline(k, nx, ny, fp32=False, target_s=12.0):
-    """The CPU restatement of the reference step on the same plate, in the
-    precision of the GPU path it stands beside (the FP64 oracle, or its FP32
-    form), rows split over the host threads as the reference splits its
-    plate into row strips over a thread pool (src/main.cpp:316-348)."""
-    from concurrent.futures import ThreadPoolExecutor
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def cpu_baseline(k, plan, nx, ny, fp32=False, target_s=12.0):
+    """The reference CPU solver's step path on the same plate, in the
+    precision of the GPU line it stands beside: per step the driver thread
+    builds 2 x threads row-strip tiles with mirrored halos, a pool of
+    `threads` workers steps them, and the driver stitches them back
+    (src/main.cpp:43-44, 125-155, 316-348, restated in C by the oracle,
+    oracle/lpbf_steptask.c).  A first single step sizes the sample to about
+    `target_s` seconds; measure() is left out (the GPU line's statistics are
+    fused, the reference's measure is a separate serial scan)."""
     import numpy as np
     from oracle import pyoracle as ora
     T = 300.0 + ora.noise_field(0, 0.0, 0.5, (ny, nx))
-    if fp32:
-        T = np.ascontiguousarray(T, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32 if fp32 else np.float64)
     kk = ora.StepConst.from_buffer_copy(bytes(k))
-    out = np.empty_like(T)
     nt = cpu_threads()
-    steps = 0
-    with ThreadPoolExecutor(nt) as pool:
-        t0 = time.perf_counter()
-        while True:
-            if fp32:
-                ora.step_threads_f32(T, kk, threads=nt, out=out, pool=pool)
-            else:
-                ora.step_threads(T, kk, 0, threads=nt, out=out, pool=pool)
-            T, out = out, T
-            steps += 1
-            el = time.perf_counter() - t0
-            if el >= target_s or steps >= 1000:
-                break
+    ntiles = 2 * nt
+    first = ora.steptask_run(T, kk, plan, 1, ntiles, nt)[2]
+    steps = int(max(1, min(2000, target_s / max(first, 1e-6))))
+    ph = ora.steptask_run(T, kk, plan[1:], steps, ntiles, nt)
+    el = ph[2]
     return {"value": nx * ny * steps / el, "unit": "cell-updates/s", "cores": nt,
             "kind": "port",
-            "sample": f"oracle {'FP32' if fp32 else 'FP64'} step (C, -O3) on {nt} host "
-                      f"threads (row ranges), {nx}x{ny} plate, {steps} steps, {el:.1f} s"}
+            "sample": f"oracle {'FP32' if fp32 else 'FP64'} step-task path (C, -O3): "
+                      f"{ntiles} row-strip tiles built, stepped by {nt} worker threads and "
+                      f"stitched per step, {nx}x{ny} plate, {steps} steps, {el:.1f} s "
+                      f"(build {ph[0]:.1f} s, wait+stitch {ph[1]:.1f} s); host "
+                      f"'{cpu_model()}', nproc {os.cpu_count()}, affinity "
+                      f"{len(os.sched_getaffinity(0))}, threads capped at 16 (one GPU's "
+                      f"host share)"}
 
 
-def traffic_for(workload, **launch):
-    """HBM bytes per launch from the committed counter summary, only when it
-    was measured on the same launch shape (e.g. depth and band rows)."""
-    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+def counters_for(workload, **launch):
+    """The committed counter summary (profiles/counters_<workload>.json) when
+    it was measured on the same launch shape (depth and band rows): HBM bytes
+    per launch (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), VALU
+    instructions per launch and the bound they show."""
+    p = os.path.join(ROOT, "profiles", f"counters_{workload}.json")
     if os.path.exists(p):
         with open(p) as f:
             t = json.load(f)
         if all(t.get(k) == v for k, v in launch.items()):
-            return t.get("bytes_per_launch")
+            return t
     return None
+
+
+def traffic_for(workload, **launch):
+    t = counters_for(workload, **launch)
+    return t.get("bytes_per_launch") if t else None
 
 
 def batch_setup(n_inst=256, n_steps=5000, seed=0):
@@ -497,22 +512,192 @@ def copy_reference(args):
                       "ms_per_copy": ms, "steps": args.steps}), flush=True)
 
 
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4   # wave64 VALU instructions/s: 1,024 SIMDs, 2.4 GHz, 4 cycles each
+
+
+def plate_measure(args, precision, min_window_ms=200.0):
+    """One plate line: autotune (unless --depth), warm-up, then timed windows
+    of exactly `steps` steps each (stream synchronised before and after every
+    window), repeated until `min_window_ms` of timed GPU work so that a short
+    driver run is not a single millisecond; value from the median window.
+    The step kernel's average launch duration comes from HIP events on its
+    stream around the full-depth launches."""
+    import numpy as np
+    import torch
+
+    import hedgehog_lpbf_amd as h
+    lib = h.lib()
+    mat, pr, dom, dt, k, plan = plate_setup(args.n, args.n)
+    nx, ny = dom.nx, dom.ny
+    cells = nx * ny
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    st = stream.cuda_stream
+    pdev = torch.from_numpy(plan).to(dev)
+    stats = torch.empty(plan.shape[0] * 7 + 7, dtype=torch.float64, device=dev)
+    tag = "plate8k_f32" if precision == 32 else "plate8k_f64"
+    print(f"[bench] {tag} {nx}x{ny} warmup={args.warmup} steps={args.steps}",
+          file=sys.stderr, flush=True)
+    depth, rows, tuned = args.depth, args.rows, None
+    if depth == 0:
+        bd, br, bms = C.c_int(), C.c_int(), C.c_double()
+        t0 = time.perf_counter()
+        if precision == 32:
+            rc = lib.lpbf_f32_autotune(k, 300.0, 0, 300.0, mat.Tm, pdev.data_ptr(),
+                                       plan.shape[0], 4, st, C.byref(bd), C.byref(br),
+                                       C.byref(bms))
+        else:
+            rc = lib.lpbf_f64_blocked_autotune(k, 0, 0, 300.0, mat.Tm, pdev.data_ptr(),
+                                               plan.shape[0], 4, st, C.byref(bd), C.byref(br),
+                                               C.byref(bms))
+        if rc != 0:
+            h.check()
+            raise RuntimeError(f"autotune failed: {rc}")
+        depth, rows = bd.value, br.value
+        tuned = {"depth": depth, "rows": rows, "ms_per_step": bms.value,
+                 "tune_s": time.perf_counter() - t0}
+        print(f"[bench] autotune: depth {depth} rows {rows} ({bms.value:.4f} ms/step)",
+              file=sys.stderr, flush=True)
+    if precision == 32:
+        ld = (nx + 3) // 4 * 4
+        a = torch.empty((ny, ld), dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        lib.lpbf_launch_noise_f32(a.data_ptr(), nx, ny, ld, 0, 0.0, 0.5, st)
+        work = torch.empty(lib.lpbf_f32_workspace_bytes(nx, ny, depth, rows) // 8 + 8,
+                           dtype=torch.float64, device=dev)
+        bytes_per_update = 8
+
+        def launch(s, d, x, y):
+            lib.lpbf_launch_step_f32(x.data_ptr(), y.data_ptr(), ld, k, 300.0, 0, 300.0,
+                                     mat.Tm, pdev.data_ptr(), s, d, rows, None, work.data_ptr(),
+                                     st)
+
+        def finalize(s, d):
+            lib.lpbf_launch_f32_stats_finalize(work.data_ptr(), nx, ny, d, rows,
+                                               stats.data_ptr() + 56 * s, st)
+        kernel_name = "k_step_f32"
+    else:
+        # FP64 parity path with temporal blocking: the field is bitwise the
+        # one `depth` single steps give
+        a = torch.empty((ny, nx), dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        lib.lpbf_launch_noise_f64(a.data_ptr(), cells, 0, 300.0, 0.5, st)
+        work = torch.empty(lib.lpbf_f64_blocked_workspace_bytes(nx, ny, depth, rows) // 8 + 8,
+                           dtype=torch.float64, device=dev)
+        bytes_per_update = 16
+
+        def launch(s, d, x, y):
+            lib.lpbf_launch_step_f64_blocked(x.data_ptr(), y.data_ptr(), k, 0, 0, 300.0, mat.Tm,
+                                             pdev.data_ptr(), s, d, rows,
+                                             stats.data_ptr() + 56 * s, work.data_ptr(), st)
+
+        def finalize(s, d):
+            pass                    # the blocked launch merges its statistics itself
+        kernel_name = "k_step_f64_tb"
+    bufs = [a, b]
+
+    def run(s0, n, events=None):
+        s = s0
+        while s < s0 + n:
+            d = min(depth, s0 + n - s)
+            if events is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+            launch(s, d, bufs[0], bufs[1])
+            if events is not None:
+                e1.record(stream)
+                events.append((e0, e1, d))
+            finalize(s, d)
+            bufs.reverse()
+            s += d
+
+    run(0, args.warmup)
+    torch.cuda.synchronize(dev)
+    h.check()
+    windows, events = [], []
+    s0 = args.warmup
+    while True:
+        if s0 + args.steps > plan.shape[0]:
+            break
+        t_start = torch.cuda.Event(enable_timing=True)
+        t_end = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize(dev)
+        t_start.record(stream)
+        run(s0, args.steps, events)
+        t_end.record(stream)
+        torch.cuda.synchronize(dev)
+        windows.append(t_start.elapsed_time(t_end))
+        s0 += args.steps
+        if sum(windows) >= min_window_ms or len(windows) >= 1000:
+            break
+    h.check()
+    if not windows:
+        raise SystemExit(f"--warmup + --steps exceed the {plan.shape[0]}-step scan plan")
+    win_ms = float(np.median(windows))
+    full = [e0.elapsed_time(e1) for e0, e1, d in events if d == depth]
+    kern_ms = float(np.mean(full)) if full else float("nan")
+    last = (h.StepStats * 1).from_buffer_copy(
+        stats.cpu().numpy().tobytes()[56 * (s0 - 1): 56 * s0])[0]
+    print(f"[bench] {len(windows)} windows of {args.steps} steps, median {win_ms:.3f} ms; "
+          f"step-kernel avg {kern_ms:.4f} ms per launch of {depth} steps; peak T "
+          f"{last.tmax:.1f} K, molten {last.molten}", file=sys.stderr, flush=True)
+
+    # roofline from the committed counter summary of the same launch shape
+    prof = counters_for(tag, depth=depth, rows=rows)
+    alg_bytes = bytes_per_update * cells * depth     # per launch, counted per step
+    compulsory = bytes_per_update * cells            # one read + one write per launch
+    kern_s = kern_ms * 1e-3
+    traffic = prof.get("bytes_per_launch") if prof else None
+    basis = "counters (FETCH_SIZE + WRITE_SIZE, copy-calibrated)" if traffic else \
+        "compulsory bytes (no counter summary for this launch shape)"
+    hbm_bytes = traffic if traffic else compulsory
+    achieved = hbm_bytes / kern_s / 1e9
+    valu = prof.get("valu_insts_per_launch") if prof else None
+    roof = {
+        "bound": prof.get("bound", "hbm") if prof else "hbm",
+        "kernel": kernel_name,
+        "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "traffic_basis": basis,
+        "valu_frac": (valu / (kern_s * VALU_ISSUE_PER_S)) if valu else None,
+        "valu_insts_per_launch": valu,
+        "alg_bytes_per_launch": alg_bytes,
+        "alg_frac": alg_bytes / kern_s / (HBM_PEAK_GBS * 1e9),
+        "kernel_ms_per_launch": kern_ms,
+        "counter_summary": prof.get("summary") if prof else None,
+    }
+    return {
+        "value": cells * args.steps / (win_ms * 1e-3),
+        "ms_per_step": win_ms / args.steps,
+        "windows": len(windows),
+        "window_ms": {"min": float(min(windows)), "median": win_ms,
+                      "max": float(max(windows))},
+        "depth": depth, "rows": rows, "tuned": tuned, "roofline": roof,
+        "k": k, "plan": plan, "nx": nx, "ny": ny,
+        "peak_T_K": last.tmax,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default: 4000 for the plates, 5000 for batch256, "
-                         "20 repetitions for case0)")
+                    help="timed steps per window (default: 4000 for the plates, 5000 for "
+                         "batch256, 20 repetitions for case0)")
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="plate8k_f32",
                     choices=["plate8k_f32", "plate8k_f64", "batch256", "case0", "ensemble",
                              "run8k_f64", "run8k_f32", "surrogate", "copy"],
-                    help="batch256: 256 uneven instances in one hipGraph launch; case0: the "
-                         "reference's published case on the device-resident path; ensemble: "
-                         "--runs manifest rows of that plate stepped together; run8k_f64 / "
-                         "run8k_f32: the device-resident run on the 8192^2 plate; copy: a "
-                         "plain device copy of one n x n FP32 plate per step, the reference "
-                         "for calibrating HBM byte counters")
+                    help="plate8k_f32 (default): the FP32 plate, with the FP64 parity path "
+                         "on the same plate in its `fp64` object; plate8k_f64: the FP64 "
+                         "line alone; batch256: 256 uneven instances in one hipGraph launch; "
+                         "case0: the reference's published case on the device-resident path; "
+                         "ensemble: --runs manifest rows of that plate stepped together; "
+                         "run8k_f64 / run8k_f32: the device-resident run on the 8192^2 plate; "
+                         "copy: a plain device copy of one n x n FP32 plate per step, the "
+                         "reference for calibrating HBM byte counters")
     ap.add_argument("--depth", type=int, default=0,
                     help="plate workloads: steps per launch (1-4); 0 = pick depth and band "
                          "rows with the library's autotuner before the warm-up")
@@ -521,18 +706,20 @@ def main():
     ap.add_argument("--n", type=int, default=8192, help="plate side (cells)")
     ap.add_argument("--runs", type=int, default=64, help="ensemble: manifest rows")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="plate8k_f32: leave out the FP64 parity-path object")
+    ap.add_argument("--min-window-ms", type=float, default=200.0,
+                    help="plate workloads: repeat the timed window of --steps steps until "
+                         "this much GPU time is timed (value from the median window)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the benchmark "
               "runs on one GPU; the multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    import torch
-
     import hedgehog_lpbf_amd as h
 
-    lib = h.lib()
+    h.lib()
     if args.workload == "batch256":
         return batch_bench(args)
     if args.workload == "case0":
@@ -549,171 +736,55 @@ def main():
         args.warmup = 400 if args.workload != "copy" else 2
     if args.workload == "copy":
         return copy_reference(args)
-    mat, pr, dom, dt, k, plan = plate_setup(args.n, args.n)
-    nx, ny = dom.nx, dom.ny
-    cells = nx * ny
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
-    st = stream.cuda_stream
-    nsteps = args.warmup + args.steps
-    assert nsteps <= plan.shape[0]
-    pdev = torch.from_numpy(plan).to(dev)
-    stats = torch.empty(nsteps * 7 + 7, dtype=torch.float64, device=dev)
-    print(f"[bench] {args.workload} {nx}x{ny} warmup={args.warmup} steps={args.steps}",
-          file=sys.stderr, flush=True)
 
-    tuned = None
-    if args.workload == "plate8k_f32":
-        depth, rows = args.depth, args.rows
-        if depth == 0:
-            bd, br, bms = C.c_int(), C.c_int(), C.c_double()
-            t0 = time.perf_counter()
-            rc = lib.lpbf_f32_autotune(k, 300.0, 0, 300.0, mat.Tm, pdev.data_ptr(), plan.shape[0],
-                                       4, st, C.byref(bd), C.byref(br), C.byref(bms))
-            if rc != 0:
-                h.check()
-                raise RuntimeError(f"autotune failed: {rc}")
-            depth, rows = bd.value, br.value
-            tuned = {"depth": depth, "rows": rows, "ms_per_step": bms.value,
-                     "tune_s": time.perf_counter() - t0}
-            print(f"[bench] autotune: depth {depth} rows {rows} ({bms.value:.4f} ms/step)",
-                  file=sys.stderr, flush=True)
-        ld = (nx + 3) // 4 * 4
-        a = torch.empty((ny, ld), dtype=torch.float32, device=dev)
-        b = torch.empty_like(a)
-        lib.lpbf_launch_noise_f32(a.data_ptr(), nx, ny, ld, 0, 0.0, 0.5, st)
-        work = torch.empty(lib.lpbf_f32_workspace_bytes(nx, ny, depth, rows) // 8 + 8,
-                           dtype=torch.float64, device=dev)
-        bytes_per_update = 8
-
-        def launch(s, d, bufs):
-            x, y = bufs
-            lib.lpbf_launch_step_f32(x.data_ptr(), y.data_ptr(), ld, k, 300.0, 0, 300.0,
-                                     mat.Tm, pdev.data_ptr(), s, d, rows, None, work.data_ptr(),
-                                     st)
-
-        def finalize(s, d):
-            lib.lpbf_launch_f32_stats_finalize(work.data_ptr(), nx, ny, d, rows,
-                                               stats.data_ptr() + 56 * s, st)
-        kernel_name = "k_step_f32"
-    else:
-        # FP64 parity path with temporal blocking: the field is bitwise the
-        # one `depth` single steps give
-        depth, rows = args.depth, args.rows
-        if depth == 0:
-            bd, br, bms = C.c_int(), C.c_int(), C.c_double()
-            t0 = time.perf_counter()
-            rc = lib.lpbf_f64_blocked_autotune(k, 0, 0, 300.0, mat.Tm, pdev.data_ptr(),
-                                               plan.shape[0], 4, st, C.byref(bd), C.byref(br),
-                                               C.byref(bms))
-            if rc != 0:
-                h.check()
-                raise RuntimeError(f"autotune failed: {rc}")
-            depth, rows = bd.value, br.value
-            tuned = {"depth": depth, "rows": rows, "ms_per_step": bms.value,
-                     "tune_s": time.perf_counter() - t0}
-            print(f"[bench] autotune: depth {depth} rows {rows} ({bms.value:.4f} ms/step)",
-                  file=sys.stderr, flush=True)
-        a = torch.empty((ny, nx), dtype=torch.float64, device=dev)
-        b = torch.empty_like(a)
-        lib.lpbf_launch_noise_f64(a.data_ptr(), cells, 0, 300.0, 0.5, st)
-        work = torch.empty(lib.lpbf_f64_blocked_workspace_bytes(nx, ny, depth, rows) // 8 + 8,
-                           dtype=torch.float64, device=dev)
-        bytes_per_update = 16
-
-        def launch(s, d, bufs):
-            x, y = bufs
-            lib.lpbf_launch_step_f64_blocked(x.data_ptr(), y.data_ptr(), k, 0, 0, 300.0, mat.Tm,
-                                             pdev.data_ptr(), s, d, rows,
-                                             stats.data_ptr() + 56 * s, work.data_ptr(), st)
-
-        def finalize(s, d):
-            pass                    # the blocked launch merges its statistics itself
-        kernel_name = "k_step_f64_tb"
-
-    bufs = [a, b]
-
-    def run(s0, n, events=None):
-        s = s0
-        while s < s0 + n:
-            d = min(depth, s0 + n - s)
-            if events is not None:
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
-            launch(s, d, bufs)
-            if events is not None:
-                e1.record(stream)
-                events.append((e0, e1, d))
-            finalize(s, d)
-            bufs.reverse()
-            s += d
-
-    run(0, args.warmup)
-    torch.cuda.synchronize(dev)
-    h.check()
-
-    events = []
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize(dev)
-    t_start.record(stream)
-    run(args.warmup, args.steps, events)
-    t_end.record(stream)
-    torch.cuda.synchronize(dev)
-    h.check()
-    total_ms = t_start.elapsed_time(t_end)
-    full = [(e0.elapsed_time(e1), d) for e0, e1, d in events if d == depth]
-    kern_ms = float(np.mean([t for t, _ in full])) if full else float("nan")
-
-    value = cells * args.steps / (total_ms * 1e-3)
-    alg_bytes = bytes_per_update * cells * depth
-    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
-    last = (h.StepStats * 1).from_buffer_copy(
-        stats.cpu().numpy().tobytes()[56 * (nsteps - 1): 56 * nsteps])[0]
-    print(f"[bench] total {total_ms:.2f} ms, step-kernel avg {kern_ms:.4f} ms per launch of "
-          f"{depth} steps, peak T {last.tmax:.1f} K, molten {last.molten}",
-          file=sys.stderr, flush=True)
-
+    prec = 32 if args.workload == "plate8k_f32" else 64
+    m = plate_measure(args, prec, args.min_window_ms)
+    fp64 = None
+    if prec == 32 and not args.no_fp64:
+        args64 = argparse.Namespace(**vars(args))
+        args64.depth, args64.rows = 0, 0
+        m64 = plate_measure(args64, 64, args.min_window_ms)
+        fp64 = {"metric": "cell-updates/s", "value": m64["value"], "unit": "cell-updates/s",
+                "dtype": "f64", "steps": args.steps, "windows": m64["windows"],
+                "ms_per_step": m64["ms_per_step"], "window_ms": m64["window_ms"],
+                "config": {"workload": "the same plate on the FP64 parity path (temporally "
+                                       "blocked, bitwise the single-step field)",
+                           "temporal_depth": m64["depth"], "band_rows": m64["rows"],
+                           "autotuned": m64["tuned"]},
+                "roofline": m64["roofline"], "cpu_baseline": None}
+    nx, ny = m["nx"], m["ny"]
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(k, nx, ny, fp32=args.workload == "plate8k_f32")
-
+        cpu = cpu_baseline(m["k"], m["plan"], nx, ny, fp32=prec == 32)
+        if fp64 is not None:
+            fp64["cpu_baseline"] = cpu_baseline(m["k"], m["plan"], nx, ny, fp32=False)
     out = {
         "metric": "cell-updates/s",
-        "value": value,
+        "value": m["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": total_ms / args.steps,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.workload == "plate8k_f32" else "f64",
+        "dtype": "f32" if prec == 32 else "f64",
         "data": "synthetic",
         "config": {
             "workload": f"{nx}x{ny} plate, dx 5 um, dt 1.2 us, 20-track raster P 250 W "
                         f"V 1.2 m/s, T0 300 K +/-0.5 K noise, "
-                        f"{'FP32 theta field' if args.workload == 'plate8k_f32' else 'FP64 field'}"
+                        f"{'FP32 theta field' if prec == 32 else 'FP64 field'}"
                         f", fused per-step stats",
-            "nx": nx, "ny": ny, "temporal_depth": depth,
-            "band_rows": rows,
-            "autotuned": tuned,
+            "nx": nx, "ny": ny, "temporal_depth": m["depth"], "band_rows": m["rows"],
+            "autotuned": m["tuned"],
+            "timed_windows": m["windows"], "window_ms": m["window_ms"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": kernel_name,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_for(args.workload, depth=depth, rows=rows),
-            "alg_bytes_per_launch": alg_bytes,
-            "kernel_ms_per_launch": kern_ms,
-        },
+        "roofline": m["roofline"],
         "cpu_baseline": cpu,
     }
+    if fp64 is not None:
+        out["fp64"] = fp64
     print(json.dumps(out), flush=True)
 
 

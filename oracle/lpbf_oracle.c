@@ -260,44 +260,78 @@ void ora_step_rows(const double* Tin, double* Tout, const StepConst* k, int phys
  * expression (step_task.hpp:93-129) evaluated in float storage and
  * arithmetic with expf.  Not a parity function: it is the FP32 CPU baseline
  * the benchmark times beside the FP32 GPU path. */
+typedef struct F32Const {
+    float inv_dx2, inv_dy2, alpha, dt, dx, dy, x_l, y_l, inv_r0_sq, S0;
+    float hConv, Tamb, Tsub, eps_sb, Ta4, Kd, loss_w;
+    int rad, nx;
+} F32Const;
+
+static F32Const f32_const(const StepConst* k) {
+    F32Const c;
+    c.inv_dx2 = (float)k->inv_dx2; c.inv_dy2 = (float)k->inv_dy2;
+    c.alpha = (float)k->alpha; c.dt = (float)k->dt;
+    c.dx = (float)k->dx; c.dy = (float)k->dy;
+    c.x_l = (float)k->x_l; c.y_l = (float)k->y_l; c.inv_r0_sq = (float)k->inv_r0_sq;
+    c.S0 = (float)(k->q_scale * k->peak_S / (k->rho_cp * k->h));
+    c.hConv = (float)k->hConv; c.Tamb = (float)k->Tamb; c.Tsub = (float)k->Tsub;
+    c.eps_sb = (float)(k->eps * ORA_SIGMA_SB);
+    c.Ta4 = c.Tamb * c.Tamb * c.Tamb * c.Tamb;
+    c.Kd = (float)(k->K / k->dSub);
+    c.loss_w = (float)(k->inv_h / k->rho_cp);
+    c.rad = k->radiation;
+    c.nx = k->nx;
+    return c;
+}
+
+/* One row y of the FP32 step from the row, the row above and the row below
+ * (ghost rows already substituted by the caller). */
+static void f32_row(const float* row, const float* up, const float* dn, float* out, int y,
+                    const F32Const* c) {
+    const int nx = c->nx;
+    const float ypos = (float)y * c->dy;
+    for (int x = 0; x < nx; ++x) {
+        const float T = row[x];
+        const float xpos = (float)x * c->dx;
+        const float Txm = (x == 0) ? T : row[x - 1];
+        const float Txp = (x == nx - 1) ? T : row[x + 1];
+        float lap_x;
+        if (x == 0)           lap_x = (Txp - T) * c->inv_dx2;
+        else if (x == nx - 1) lap_x = (Txm - T) * c->inv_dx2;
+        else                  lap_x = (Txp - 2.0f * T + Txm) * c->inv_dx2;
+        const float lap_y = (dn[x] - 2.0f * T + up[x]) * c->inv_dy2;
+        const float r2 = (xpos - c->x_l) * (xpos - c->x_l) + (ypos - c->y_l) * (ypos - c->y_l);
+        const float S = c->S0 * expf(-2.0f * r2 * c->inv_r0_sq);
+        float q_top = c->hConv * (T - c->Tamb);
+        if (c->rad) q_top += c->eps_sb * (T * T * T * T - c->Ta4);
+        const float loss = (q_top + c->Kd * (T - c->Tsub)) * c->loss_w;
+        out[x] = T + c->dt * (c->alpha * (lap_x + lap_y) + S - loss);
+    }
+}
+
 void ora_step_rows_f32(const float* Tin, float* Tout, const StepConst* k, int y0, int y1) {
     const int nx = k->nx, ny = k->ny;
-    const float inv_dx2 = (float)k->inv_dx2, inv_dy2 = (float)k->inv_dy2;
-    const float alpha = (float)k->alpha, dt = (float)k->dt;
-    const float dx = (float)k->dx, dy = (float)k->dy;
-    const float x_l = (float)k->x_l, y_l = (float)k->y_l, inv_r0_sq = (float)k->inv_r0_sq;
-    const float S0 = (float)(k->q_scale * k->peak_S / (k->rho_cp * k->h));
-    const float hConv = (float)k->hConv, Tamb = (float)k->Tamb, Tsub = (float)k->Tsub;
-    const float eps_sb = (float)(k->eps * ORA_SIGMA_SB);
-    const float Ta4 = Tamb * Tamb * Tamb * Tamb;
-    const float Kd = (float)(k->K / k->dSub);
-    const float loss_w = (float)(k->inv_h / k->rho_cp);
-    const int rad = k->radiation;
+    const F32Const c = f32_const(k);
     if (y0 < 0) y0 = 0;
     if (y1 > ny) y1 = ny;
     for (int y = y0; y < y1; ++y) {
-        const float ypos = (float)y * dy;
-        for (int x = 0; x < nx; ++x) {
-            const long long c = (long long)y * nx + x;
-            const float T = Tin[c];
-            const float xpos = (float)x * dx;
-            const float Txm = (x == 0) ? T : Tin[c - 1];
-            const float Txp = (x == nx - 1) ? T : Tin[c + 1];
-            const float Tym = (y == 0) ? T : Tin[c - nx];
-            const float Typ = (y == ny - 1) ? T : Tin[c + nx];
-            float lap_x;
-            if (x == 0)           lap_x = (Txp - T) * inv_dx2;
-            else if (x == nx - 1) lap_x = (Txm - T) * inv_dx2;
-            else                  lap_x = (Txp - 2.0f * T + Txm) * inv_dx2;
-            const float lap_y = (Typ - 2.0f * T + Tym) * inv_dy2;
-            const float r2 = (xpos - x_l) * (xpos - x_l) + (ypos - y_l) * (ypos - y_l);
-            const float S = S0 * expf(-2.0f * r2 * inv_r0_sq);
-            float q_top = hConv * (T - Tamb);
-            if (rad) q_top += eps_sb * (T * T * T * T - Ta4);
-            const float loss = (q_top + Kd * (T - Tsub)) * loss_w;
-            Tout[c] = T + dt * (alpha * (lap_x + lap_y) + S - loss);
-        }
+        const float* row = Tin + (size_t)y * nx;
+        const float* up = (y == 0) ? row : row - nx;
+        const float* dn = (y == ny - 1) ? row : row + nx;
+        f32_row(row, up, dn, Tout + (size_t)y * nx, y, &c);
     }
+}
+
+/* ora_step_tile in FP32 (baseline physics): the tile's halo rows carry the
+ * neighbour or mirror rows, so the owned rows equal ora_step_rows_f32's. */
+void ora_step_tile_f32(const float* tile, float* out, int owned, int y0, const StepConst* k) {
+    const int nx = k->nx, rows = owned + 2;
+    const F32Const c = f32_const(k);
+    memcpy(out, tile, sizeof(float) * (size_t)nx);
+    memcpy(out + (size_t)(rows - 1) * nx, tile + (size_t)(rows - 1) * nx,
+           sizeof(float) * (size_t)nx);
+    for (int r = 1; r < rows - 1; ++r)
+        f32_row(tile + (size_t)r * nx, tile + (size_t)(r - 1) * nx, tile + (size_t)(r + 1) * nx,
+                out + (size_t)r * nx, y0 + (r - 1), &c);
 }
 
 void ora_step_ex(const double* Tin, double* Tout, const StepConst* k,

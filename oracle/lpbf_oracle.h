@@ -89,6 +89,9 @@ void   ora_step_rows(const double* Tin, double* Tout, const StepConst* k, int ph
 /* The baseline step of ora_step_rows in FP32 storage and arithmetic (the
  * FP32 CPU baseline; mirror edges). */
 void   ora_step_rows_f32(const float* Tin, float* Tout, const StepConst* k, int y0, int y1);
+/* ora_step_tile in FP32 (baseline physics): same tile shape in float. */
+void   ora_step_tile_f32(const float* tile, float* out, int owned_rows, int y0,
+                         const StepConst* k);
 
 /* Row-strip tile step (step_task.hpp:54-206).  `tile` holds rows+2 rows of
  * nx values: halo row 0, owned rows 1..rows, halo row rows+1.  Owned rows
@@ -162,6 +165,21 @@ void   ora_track_plan(double V, double dt, double Lx, double Ly, int n_steps,
 void   ora_run_plan(const StepConst* k0, int phys, int bc, double T_bc,
                     double Tm, const LaserTick* plan, int n_steps, double* T,
                     StepStats* stats);
+
+/* The reference CPU solver's time loop with its step-task graph
+ * (src/main.cpp:316-348, include/step_task.hpp:54-206): per step the
+ * calling thread builds `ntiles` row-strip tiles with mirrored halos (a new
+ * buffer each, main.cpp:125-155), `nthreads` worker threads execute them
+ * (each writes a new buffer and drops the input, step_task.hpp:78-84,
+ * 190-191), and the calling thread stitches the owned rows back as the
+ * tiles complete, in tile order.  prec 64: T is double, phys 0/1; prec 32:
+ * T is float, baseline physics (ora_step_tile_f32).  The laser of step s is
+ * plan[s].  T (nx*ny, row-major) is the initial field on entry and the
+ * final field on exit.  phase_s (may be NULL) receives the seconds spent
+ * building tiles, waiting for and stitching results, and in total.
+ * Returns 0, or -1 for invalid arguments or a failed allocation. */
+int    ora_steptask_run(int prec, const StepConst* k, int phys, const LaserTick* plan,
+                        int n_steps, int ntiles, int nthreads, void* T, double* phase_s);
 
 #ifdef __cplusplus
 }

@@ -130,6 +130,9 @@ def load() -> C.CDLL:
                         _DP, _DP, _DP, _DP, _DP]),
         "ora_run_plan": (v, [C.c_void_p, i, i, d, d, C.c_void_p, i, _DP,
                              C.c_void_p]),
+        "ora_step_tile_f32": (v, [C.c_void_p, C.c_void_p, i, i, C.c_void_p]),
+        "ora_steptask_run": (i, [i, C.c_void_p, i, C.c_void_p, i, i, i, C.c_void_p,
+                                 C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -223,6 +226,23 @@ def step_threads(T: np.ndarray, k: StepConst, phys: int = 0, threads: int = 1,
     else:
         list(pool.map(part, range(threads)))
     return out
+
+
+def steptask_run(T: np.ndarray, k: StepConst, plan: np.ndarray, n_steps: int, ntiles: int,
+                 nthreads: int, phys: int = 0):
+    """ora_steptask_run: n_steps of the reference CPU solver's loop (tiles
+    built, stepped by a thread pool, stitched back) on T in place; float64 or
+    float32 (baseline physics).  Returns the seconds spent building tiles,
+    waiting for and stitching results, and in total."""
+    assert T.flags.c_contiguous and T.dtype in (np.float64, np.float32)
+    p = np.ascontiguousarray(plan, dtype=np.float64)
+    assert p.shape[0] >= n_steps and p.shape[1] == 3
+    ph = (C.c_double * 3)()
+    rc = lib().ora_steptask_run(32 if T.dtype == np.float32 else 64, C.byref(k), phys,
+                                p.ctypes.data, n_steps, ntiles, nthreads, T.ctypes.data, ph)
+    if rc != 0:
+        raise RuntimeError("ora_steptask_run failed")
+    return tuple(ph)
 
 
 def step_ex(T: np.ndarray, k: StepConst, phys: int = 0, bc: int = 0,

@@ -453,3 +453,38 @@ def test_fp32_cpu_baseline_step_within_fp32_rounding(ora):
     # the CPU: radiation, substrate loss or source dropped, lap_y scaled
     # by 1.001, each fail the first bound)
     assert np.abs(ref - T).min() > 0.5
+
+
+@pytest.mark.parametrize("ntiles,nthreads", [(1, 1), (6, 3), (16, 8), (40, 5)])
+def test_steptask_run_equals_full_grid_steps(ora, ntiles, nthreads):
+    """The CPU baseline's step-task loop (tiles built, stepped by a thread
+    pool, stitched in order) gives exactly the full-grid oracle steps, in
+    FP64 (both physics) and FP32, for any tile and thread count (tiling
+    independence, lpbf_cuda.cu:14-18)."""
+    s = ora.defaults("0")
+    s.dom.nx, s.dom.ny = 97, 37
+    k = ora.make_const(s, ora.dt_cfl(s))
+    n = 7
+    plan = ora.track_plan(3.0, k.dt, s.dom.Lx, s.dom.Ly, n)
+    rng = np.random.default_rng(ntiles)
+    T0 = 300.0 + 1500.0 * rng.random((37, 97))
+    for phys in (0, 1):
+        want = T0.copy()
+        for q in range(n):
+            kk = ora.StepConst.from_buffer_copy(bytes(k))
+            kk.x_l, kk.y_l, kk.q_scale = plan[q]
+            want = ora.step(want, kk, phys)
+        got = T0.copy()
+        ph = ora.steptask_run(got, k, plan, n, ntiles, nthreads, phys)
+        assert np.array_equal(got, want), phys
+        assert ph[2] >= ph[0] >= 0.0 and ph[2] >= ph[1] >= 0.0
+    want = np.ascontiguousarray(T0, dtype=np.float32)
+    for q in range(n):
+        kk = ora.StepConst.from_buffer_copy(bytes(k))
+        kk.x_l, kk.y_l, kk.q_scale = plan[q]
+        want = ora.step_threads_f32(want, kk, threads=1)
+    got = np.ascontiguousarray(T0, dtype=np.float32)
+    ora.steptask_run(got, k, plan, n, ntiles, nthreads)
+    assert np.array_equal(got, want)
+    with pytest.raises(RuntimeError):
+        ora.steptask_run(np.ascontiguousarray(T0, dtype=np.float32), k, plan, n, 4, 2, phys=1)
