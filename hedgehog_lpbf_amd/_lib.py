@@ -191,6 +191,15 @@ def lib() -> C.CDLL:
     """Load the library (once).  Raises if it has not been built."""
     global _LIB
     if _LIB is None:
+        # torch bundles its own HIP runtime (soname libamdhip64.so.7, loaded
+        # through the name libamdhip64.so); loaded first, it is the runtime
+        # this library binds to as well.  Loaded after this library, it would
+        # come in as a second runtime next to the system one, and device
+        # memory from one is unusable by the other.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with __graft_entry__.build() or "
