@@ -17,6 +17,8 @@ LPBF_BC_DIRICHLET = 1
 LPBF_PHYS_BASELINE = 0
 LPBF_PHYS_EXTENDED = 1
 LPBF_ERR_INVALID_ARGUMENT = 1000
+LPBF_ERR_SCRATCH_LIMIT = 1001
+LPBF_F32_NOALLOC_SCRATCH_BYTES = 4 << 20
 
 
 class StepConst(C.Structure):
@@ -125,6 +127,8 @@ SIGNATURES = {
     "lpbf_f32_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "lpbf_launch_step_f32": (_v, [_vp, _vp, _i, _KP, _d, _i, _d, _d, _vp, _i, _i, _i,
                                   _vp, _vp, _vp]),
+    "lpbf_launch_step_f32_noalloc": (_v, [_vp, _vp, _i, _KP, _d, _i, _d, _d, _vp, _i, _i, _i,
+                                          _vp, _vp]),
     "lpbf_launch_f32_stats_finalize": (_v, [_vp, _i, _i, _i, _i, _vp, _vp]),
     "lpbf_f32_autotune": (_i, [_KP, _d, _i, _d, _d, _vp, _i, _i, _vp, C.POINTER(C.c_int),
                                C.POINTER(C.c_int), C.POINTER(C.c_double)]),

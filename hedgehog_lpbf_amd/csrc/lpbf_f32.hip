@@ -2,6 +2,8 @@
 // FP32 performance path for gfx950, single field: the temporally blocked
 // step kernel (tile body in lpbf_f32_kernel.hpp), its statistics merge,
 // noise, conversions, the hot column, the autotuner and the C ABI.
+#include <mutex>
+
 #include "lpbf_f32_kernel.hpp"
 
 namespace lpbf_amd {
@@ -87,6 +89,72 @@ void launch_k(const F32Args& a, const LaserTick* plan, int step0, hipStream_t s)
     else launch_kr<K, false>(a, plan, step0, s);
 }
 
+bool step_f32_args_ok(const float* theta_in, const float* theta_out, int ld, const StepConst* k,
+                      int bc, const LaserTick* d_plan, int step0, int depth, int rows) {
+    return k && f32_shape_ok(k->nx, k->ny) && theta_in && theta_out && theta_in != theta_out &&
+           ld >= k->nx && !(ld & 3) && !(reinterpret_cast<uintptr_t>(theta_in) & 15) &&
+           !(reinterpret_cast<uintptr_t>(theta_out) & 15) && d_plan && step0 >= 0 &&
+           depth >= 1 && depth <= kMaxDepth && rows >= 0 &&
+           (bc == LPBF_BC_MIRROR || bc == LPBF_BC_DIRICHLET);
+}
+
+// The step launch and, with d_stats, its statistics merge; `work` holds the
+// per-workgroup records (every record is written before the merge reads it).
+void step_f32_launch(const float* theta_in, float* theta_out, int ld, const StepConst* k,
+                     double T_ref, int bc, double T_bc, double Tm, const LaserTick* d_plan,
+                     int step0, int depth, int rows, StepStats* d_stats, void* work,
+                     hipStream_t stream) {
+    const F32Args a = make_args(theta_in, theta_out, ld, *k, T_ref, bc, T_bc, Tm, depth, rows,
+                                work);
+    switch (depth) {
+        case 1: launch_k<1>(a, d_plan, step0, stream); break;
+        case 2: launch_k<2>(a, d_plan, step0, stream); break;
+        case 3: launch_k<3>(a, d_plan, step0, stream); break;
+        default: launch_k<4>(a, d_plan, step0, stream); break;
+    }
+    check_launch();
+    if (d_stats) {
+        hipLaunchKernelGGL(k_stats_finalize_levels, dim3(depth), dim3(kThreads), 0, stream, a.part,
+                           a.nwg, k->nx, k->ny, d_stats);
+        check_launch();
+    }
+}
+
+// Workspace of lpbf_launch_step_f32_noalloc: one array per device, for the
+// callers that cannot allocate device memory.  In host code `scratch` names
+// a host-side stand-in; its device address comes from hipGetSymbolAddress,
+// looked up once per device (with that device current) and kept here.
+__device__ __align__(256) unsigned char scratch[LPBF_F32_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// The current device's scratch address, or nullptr (error recorded).
+void* scratch_of_current_device() {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        record_error(static_cast<int>(e));
+        return nullptr;
+    }
+    if (dev < 0 || dev >= kMaxDevices) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            record_error(e != hipSuccess ? static_cast<int>(e) : LPBF_ERR_INVALID_ARGUMENT);
+            return nullptr;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    return g_scratch_dev[dev];
+}
+
 }  // namespace
 }  // namespace lpbf_amd
 
@@ -110,28 +178,33 @@ void lpbf_launch_step_f32(const float* theta_in, float* theta_out, int ld, const
                           double T_ref, int bc, double T_bc, double Tm, const LaserTick* d_plan,
                           int step0, int depth, int rows, StepStats* d_stats, void* d_work,
                           hipStream_t stream) {
-    if (!k || !f32_shape_ok(k->nx, k->ny) || !theta_in || !theta_out || theta_in == theta_out ||
-        ld < k->nx || (ld & 3) || (reinterpret_cast<uintptr_t>(theta_in) & 15) ||
-        (reinterpret_cast<uintptr_t>(theta_out) & 15) || !d_plan || step0 < 0 || depth < 1 ||
-        depth > kMaxDepth || rows < 0 || !d_work ||
-        (bc != LPBF_BC_MIRROR && bc != LPBF_BC_DIRICHLET)) {
+    if (!step_f32_args_ok(theta_in, theta_out, ld, k, bc, d_plan, step0, depth, rows) ||
+        !d_work) {
         record_error(LPBF_ERR_INVALID_ARGUMENT);
         return;
     }
-    const F32Args a = make_args(theta_in, theta_out, ld, *k, T_ref, bc, T_bc, Tm, depth, rows,
-                                d_work);
-    switch (depth) {
-        case 1: launch_k<1>(a, d_plan, step0, stream); break;
-        case 2: launch_k<2>(a, d_plan, step0, stream); break;
-        case 3: launch_k<3>(a, d_plan, step0, stream); break;
-        default: launch_k<4>(a, d_plan, step0, stream); break;
+    step_f32_launch(theta_in, theta_out, ld, k, T_ref, bc, T_bc, Tm, d_plan, step0, depth, rows,
+                    d_stats, d_work, stream);
+}
+
+void lpbf_launch_step_f32_noalloc(const float* theta_in, float* theta_out, int ld,
+                                  const StepConst* k, double T_ref, int bc, double T_bc,
+                                  double Tm, const LaserTick* d_plan, int step0, int depth,
+                                  int rows, StepStats* d_stats, hipStream_t stream) {
+    if (!step_f32_args_ok(theta_in, theta_out, ld, k, bc, d_plan, step0, depth, rows)) {
+        record_error(LPBF_ERR_INVALID_ARGUMENT);
+        return;
     }
-    check_launch();
-    if (d_stats) {
-        hipLaunchKernelGGL(k_stats_finalize_levels, dim3(depth), dim3(kThreads), 0, stream, a.part,
-                           a.nwg, k->nx, k->ny, d_stats);
-        check_launch();
+    const Geometry g = geometry(k->nx, k->ny, depth, rows);
+    const size_t need = sizeof(StepStats) * static_cast<size_t>(g.nwg) * static_cast<size_t>(depth);
+    if (need > LPBF_F32_NOALLOC_SCRATCH_BYTES) {
+        record_error(LPBF_ERR_SCRATCH_LIMIT);
+        return;
     }
+    void* work = scratch_of_current_device();
+    if (!work) return;
+    step_f32_launch(theta_in, theta_out, ld, k, T_ref, bc, T_bc, Tm, d_plan, step0, depth, rows,
+                    d_stats, work, stream);
 }
 
 void lpbf_launch_f32_stats_finalize(const void* d_work, int nx, int ny, int depth, int rows,

@@ -65,6 +65,7 @@ void lpbf_launch_extract_col(const double* T, int nx, int ny,
 /* First error recorded by any lpbf_* call since the last query (a hipError_t
  * value, or LPBF_ERR_INVALID_ARGUMENT), then cleared.  0 = success. */
 #define LPBF_ERR_INVALID_ARGUMENT 1000
+#define LPBF_ERR_SCRATCH_LIMIT    1001   /* lpbf_launch_step_f32_noalloc: workspace too large */
 int         lpbf_get_last_error(void);
 const char* lpbf_error_string(int err);
 
@@ -236,6 +237,24 @@ void   lpbf_launch_step_f32(const float* theta_in, float* theta_out, int ld,
                             double T_bc, double Tm, const LaserTick* d_plan,
                             int step0, int depth, int rows, StepStats* d_stats,
                             void* d_work, hipStream_t stream);
+
+/* lpbf_launch_step_f32 without a caller workspace, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * per-workgroup statistics records live in a fixed device array of the
+ * library, LPBF_F32_NOALLOC_SCRATCH_BYTES per device, held until the process
+ * ends; no call allocates or frees memory.  Same arguments, kernels and
+ * results as lpbf_launch_step_f32.  A launch whose records
+ * (depth * workgroups * sizeof(StepStats)) exceed the array is refused with
+ * LPBF_ERR_SCRATCH_LIMIT (the 8192^2 plate at depth 4 needs ~0.35 MB with
+ * the built-in band height).  Every call on a device shares that device's
+ * array: calls that overlap on one device (other streams, other host
+ * threads) race; calls in order on one stream do not. */
+#define LPBF_F32_NOALLOC_SCRATCH_BYTES (4u << 20)
+void   lpbf_launch_step_f32_noalloc(const float* theta_in, float* theta_out, int ld,
+                                    const StepConst* k, double T_ref, int bc, double T_bc,
+                                    double Tm, const LaserTick* d_plan, int step0, int depth,
+                                    int rows, StepStats* d_stats, hipStream_t stream);
 
 /* The per-step merge that lpbf_launch_step_f32 runs when d_stats != NULL,
  * as its own call: after a step launch with d_stats == NULL and the same

@@ -178,3 +178,20 @@ def test_f64_blocked_workspace_covers_shorter_launches(nx, ny, rows):
     # the cases where depth-D sizing alone was short
     if (nx, ny, rows) == (240, 120, 0):
         assert _tb_records(240, 120, 3, 0) > _tb_records(240, 120, 4, 0)
+
+
+def test_noalloc_step_refuses_a_launch_beyond_its_scratch():
+    """lpbf_launch_step_f32_noalloc checks its workspace need on the host and
+    refuses, before any device work, a launch whose records exceed the fixed
+    scratch (here 2-row bands of a 2^15-wide, 2^14-tall plate at depth 4)."""
+    lib = h.lib()
+    lib.lpbf_get_last_error()
+    k = h.StepConst()
+    k.nx, k.ny = 1 << 15, 1 << 14
+    lib.lpbf_launch_step_f32_noalloc(16, 1 << 20, k.nx, k, 300.0, 0, 300.0, 1609.0, 16, 0, 4, 8,
+                                     None, None)
+    assert lib.lpbf_get_last_error() == h._lib.LPBF_ERR_SCRATCH_LIMIT
+    assert "scratch" in lib.lpbf_error_string(h._lib.LPBF_ERR_SCRATCH_LIMIT).decode()
+    lib.lpbf_launch_step_f32_noalloc(16, 16, 64, k, 300.0, 0, 300.0, 1609.0, 16, 0, 1, 0,
+                                     None, None)                  # in == out
+    assert lib.lpbf_get_last_error() == h._lib.LPBF_ERR_INVALID_ARGUMENT
