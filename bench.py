@@ -113,8 +113,9 @@ def counters_for(workload, **launch):
     if os.path.exists(p):
         with open(p) as f:
             t = json.load(f)
-        if all(t.get(k) == v for k, v in launch.items()):
-            return t
+        for sh in t.get("shapes", [t]):     # one launch shape, or a list of them
+            if all(sh.get(k) == v for k, v in launch.items()):
+                return dict(sh, summary=t.get("summary"))
     return None
 
 
