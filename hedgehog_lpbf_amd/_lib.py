@@ -82,6 +82,34 @@ assert C.sizeof(StepConst) == 216 and C.sizeof(Partial) == 24
 assert C.sizeof(StepRecord) == 80 and C.sizeof(F32Instance) == 288
 assert C.sizeof(LaserTick) == 24 and C.sizeof(StepStats) == 56
 
+
+class MrWindow(C.Structure):
+    """include/lpbf_multirate.h lpbf_mr_window."""
+    _fields_ = [("fine", C.c_void_p), ("c0", C.c_void_p), ("c1", C.c_void_p),
+                ("plan", C.c_void_p), ("dx", C.c_double), ("dy", C.c_double),
+                ("inv_r0_sq", C.c_double), ("ldf", C.c_int), ("fx", C.c_int), ("fy", C.c_int),
+                ("ox", C.c_int), ("oy", C.c_int), ("ldc", C.c_int), ("nxc", C.c_int),
+                ("nyc", C.c_int), ("step", C.c_int), ("ld1", C.c_int), ("c1x", C.c_int),
+                ("c1y", C.c_int), ("pad_", C.c_int)]
+
+
+class MrConfig(C.Structure):
+    """include/lpbf_multirate.h lpbf_mr_config."""
+    _fields_ = [("k", StepConst), ("T_ref", C.c_double), ("tol_K", C.c_double),
+                ("n_steps", C.c_int), ("recheck_every", C.c_int), ("overlap", C.c_int),
+                ("use_network", C.c_int), ("coarse_depth", C.c_int), ("coarse_rows", C.c_int)]
+
+
+class MrCounters(C.Structure):
+    """include/lpbf_multirate.h lpbf_mr_counters."""
+    _fields_ = [("blocks", C.c_longlong), ("calls", C.c_longlong), ("applied", C.c_longlong),
+                ("rechecks", C.c_longlong), ("fallbacks", C.c_longlong),
+                ("fd_blocks", C.c_longlong), ("max_recheck_K", C.c_double),
+                ("max_applied_recheck_K", C.c_double)]
+
+
+assert C.sizeof(MrWindow) == 112 and C.sizeof(MrConfig) == 256 and C.sizeof(MrCounters) == 64
+
 _vp, _i, _d, _v = C.c_void_p, C.c_int, C.c_double, None
 _ll, _ull, _sz = C.c_longlong, C.c_ulonglong, C.c_size_t
 _DP = C.POINTER(C.c_double)
@@ -156,9 +184,17 @@ SIGNATURES = {
     "lpbf_sg_loss_grad": (_v, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp]),
     "lpbf_sg_adam": (_v, [_vp, _vp, _vp, _vp, _i, _i, C.c_float, C.c_float, C.c_float,
                           C.c_float, _vp]),
-    "lpbf_sg_gather": (_v, [_vp, _i, _vp, _vp, _vp]),
-    "lpbf_sg_target": (_v, [_vp, _i, _vp, _vp, _vp]),
-    "lpbf_sg_apply": (_v, [_vp, _i, _vp, _vp, _i, _vp, _vp, _vp]),
+    # two-level run (include/lpbf_multirate.h)
+    "lpbf_mr_restrict": (_v, [_vp, _i, _i, _i, _vp, _i, _vp]),
+    "lpbf_mr_window_origins": (_i, [_vp, _i, _i, _i, _d, _d, _vp, _vp]),
+    "lpbf_mr_gather": (_v, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "lpbf_mr_target": (_v, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "lpbf_mr_create": (_i, [C.POINTER(MrConfig), _vp, _vp, _vp, C.POINTER(C.c_void_p)]),
+    "lpbf_mr_destroy": (_v, [_vp]),
+    "lpbf_mr_advance": (_i, [_vp, _i]),
+    "lpbf_mr_field": (_i, [_vp, _vp]),
+    "lpbf_mr_get_counters": (_i, [_vp, C.POINTER(MrCounters)]),
+    "lpbf_mr_recheck_diffs": (_i, [_vp, _vp, _i]),
     # device-resident run (include/lpbf_run.h)
     "lpbf_run_create": (_i, [_KP, _i, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _i,
                              C.POINTER(RunOptions), C.POINTER(_vp)]),

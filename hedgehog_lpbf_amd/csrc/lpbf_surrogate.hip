@@ -572,80 +572,6 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// ---- windows --------------------------------------------------------------------
-
-__global__ __launch_bounds__(kT) void k_gather(const lpbf_sg_window* __restrict__ win,
-                                               uint16_t* __restrict__ x0, float* __restrict__ T0) {
-    const int w = blockIdx.y;
-    const int p = blockIdx.x * kT + threadIdx.x;
-    if (p >= kPix) return;
-    const lpbf_sg_window& W = win[w];
-    const int i = p / kW, j = p % kW;
-    const int fx = W.ox + j, fy = W.oy + i;
-    // a window reaching outside its field reads zeros there
-    const bool in = fx >= 0 && fx < W.nx && fy >= 0 && fy < W.ny;
-    const float th = in ? W.theta[static_cast<size_t>(fy) * W.ld + fx] : 0.f;
-    T0[static_cast<size_t>(w) * kPix + p] = th;
-    const double xpos = fx * W.dx, ypos = fy * W.dy;
-    double src = 0.0;
-    for (int s = 0; s < LPBF_SG_STEPS; ++s) {
-        const LaserTick t = W.plan[W.step + s];
-        if (t.q_scale == 0.0) continue;
-        const double ddx = xpos - t.x_l, ddy = ypos - t.y_l;
-        src += t.q_scale * exp(-2.0 * (ddx * ddx + ddy * ddy) * W.inv_r0_sq);
-    }
-    src *= 1.0 / LPBF_SG_STEPS;
-    const uint16_t c0 = f2bf(th / 3000.f), c1 = f2bf(static_cast<float>(src));
-    const uint16_t c2 = f2bf(static_cast<float>(W.P / 400.0)), c3 = f2bf(static_cast<float>(W.V / 2.0));
-    uint4* out = reinterpret_cast<uint4*>(x0 + (static_cast<size_t>(w) * kPix + p) * kC);
-    out[0] = make_uint4(c0 | (uint32_t(c1) << 16), c2 | (uint32_t(c3) << 16), 0, 0);
-    out[1] = make_uint4(0, 0, 0, 0);
-    out[2] = make_uint4(0, 0, 0, 0);
-    out[3] = make_uint4(0, 0, 0, 0);
-}
-
-__global__ __launch_bounds__(kT) void k_target(const lpbf_sg_window* __restrict__ win,
-                                               const float* __restrict__ T0,
-                                               float* __restrict__ tgt) {
-    const int w = blockIdx.y;
-    const int p = blockIdx.x * kT + threadIdx.x;
-    if (p >= kPix) return;
-    const lpbf_sg_window& W = win[w];
-    const int i = p / kW, j = p % kW;
-    const int fx = W.ox + j, fy = W.oy + i;
-    const bool in = fx >= 0 && fx < W.nx && fy >= 0 && fy < W.ny;
-    const float th = in ? W.theta[static_cast<size_t>(fy) * W.ld + fx] : 0.f;
-    const size_t q = static_cast<size_t>(w) * kPix + p;
-    tgt[q] = (th - T0[q]) / 3000.f;
-}
-
-__global__ __launch_bounds__(kT) void k_apply(const lpbf_sg_window* __restrict__ win,
-                                              const float* __restrict__ T0,
-                                              const float* __restrict__ y, int overlap,
-                                              const int* __restrict__ skip,
-                                              float* __restrict__ maxdiff) {
-    const int w = blockIdx.y;
-    const int p = blockIdx.x * kT + threadIdx.x;
-    if (p >= kPix) return;
-    const lpbf_sg_window& W = win[w];
-    const int i = p / kW, j = p % kW;
-    const int fx = W.ox + j, fy = W.oy + i;
-    const bool in = fx >= 0 && fx < W.nx && fy >= 0 && fy < W.ny;
-    const size_t q = static_cast<size_t>(w) * kPix + p;
-    float* cell = W.theta_out + (in ? static_cast<size_t>(fy) * W.ld + fx : 0);
-    const float pred = T0[q] + 3000.f * y[q];
-    const float cur = in ? *cell : pred;
-    if (maxdiff) {
-        float d = fabsf(pred - cur);
-        d = wave_max_f(d);
-        if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<int*>(maxdiff + w), __float_as_int(d));
-    }
-    if (!in || (skip && skip[w])) return;
-    const int e = min(min(i, j), min(kW - 1 - i, kW - 1 - j));
-    const float a = overlap > 0 ? fminf(1.f, static_cast<float>(e) / overlap) : 1.f;
-    *cell = a * pred + (1.f - a) * cur;
-}
-
 }  // namespace
 }  // namespace lpbf_amd
 
@@ -824,36 +750,5 @@ void lpbf_sg_adam(float* d_params, const float* d_grad, float* d_m, float* d_v, 
     check_launch();
 }
 
-void lpbf_sg_gather(const lpbf_sg_window* d_win, int n, void* d_x0, float* d_T0,
-                    hipStream_t stream) {
-    if (!d_win || n < 1 || n > 65535 || !d_x0 || !d_T0) {
-        record_error(LPBF_ERR_INVALID_ARGUMENT);
-        return;
-    }
-    hipLaunchKernelGGL(k_gather, dim3(kPix / kT, n), dim3(kT), 0, stream, d_win,
-                       static_cast<uint16_t*>(d_x0), d_T0);
-    check_launch();
-}
-
-void lpbf_sg_target(const lpbf_sg_window* d_win, int n, const float* d_T0, float* d_target,
-                    hipStream_t stream) {
-    if (!d_win || n < 1 || n > 65535 || !d_T0 || !d_target) {
-        record_error(LPBF_ERR_INVALID_ARGUMENT);
-        return;
-    }
-    hipLaunchKernelGGL(k_target, dim3(kPix / kT, n), dim3(kT), 0, stream, d_win, d_T0, d_target);
-    check_launch();
-}
-
-void lpbf_sg_apply(const lpbf_sg_window* d_win, int n, const float* d_T0, const float* d_y,
-                   int overlap, const int* d_skip, float* d_maxdiff, hipStream_t stream) {
-    if (!d_win || n < 1 || n > 65535 || !d_T0 || !d_y || overlap < 0 || overlap > kW / 2) {
-        record_error(LPBF_ERR_INVALID_ARGUMENT);
-        return;
-    }
-    hipLaunchKernelGGL(k_apply, dim3(kPix / kT, n), dim3(kT), 0, stream, d_win, d_T0, d_y, overlap,
-                       d_skip, d_maxdiff);
-    check_launch();
-}
 
 }  // extern "C"

@@ -5,7 +5,8 @@
 // pushed through lpbf::Graph to StepTileTask workers (each stepping its tile
 // on the GPU), stitched back, the laser advanced by the reference law and
 // the global metrics measured; metrics.csv / meta.json follow the reference
-// formats (main.cpp:281-282, 364-379, 428-467).
+// formats (main.cpp:281-282, 364-379, 428-467), and perstep.csv carries the
+// reference's per-step phase timers (main.cpp:298-310, 382-401).
 //
 //   lpbf_tiles --case=0 --threads=4 --tiles=8 --tend=1.5e-3 --snap-every=200
 //              [--physics=base|ext] [--kmult=F] [--P= --V= --r0= --eta=
@@ -160,10 +161,24 @@ int main(int argc, char** argv) {
 
     std::ofstream csv(dir + "/metrics.csv");
     csv << "step,t_us,x_laser_um,peakT_K,W_um\n";
+    // per-step phase timers, the columns of the reference driver's
+    // perstep.csv (src/main.cpp:298-310, 382-401): build the tiles, push
+    // them, wait for results, stitch them, advance the laser, measure, dump
+    std::ofstream perstep(dir + "/perstep.csv");
+    perstep << "step,build_us,push_us,wait_us,stitch_us,advance_us,metrics_us,snapshot_us,"
+               "total_us\n";
+    using clk = std::chrono::steady_clock;
+    auto us = [](clk::time_point a, clk::time_point b) {
+        return std::chrono::duration<double, std::micro>(b - a).count();
+    };
+    enum { kBuild, kPush, kWait, kStitch, kAdvance, kMetrics, kSnapshot, kPhases };
+    double phase_us[kPhases] = {};
     double peak_T = pr.Tamb, peak_W = 0.0, steady = 0.0;
-    const auto wall0 = std::chrono::steady_clock::now();
+    const auto wall0 = clk::now();
     for (int s = 0; s < n_steps; ++s) {
         const Tick& tk = plan[static_cast<size_t>(s)];
+        const auto tb0 = clk::now();
+        std::vector<std::shared_ptr<HaloTile>> tiles(static_cast<size_t>(ntiles));
         for (int i = 0; i < ntiles; ++i) {
             auto t = std::make_shared<HaloTile>();
             t->idx = i; t->y0 = y0[i]; t->y1 = y1[i]; t->nx = dom.nx;
@@ -177,22 +192,36 @@ int main(int argc, char** argv) {
             std::copy(row(y0[i]), row(y1[i] + 1), t->data.begin() + dom.nx);
             std::copy(row(bot), row(bot) + dom.nx,
                       t->data.begin() + static_cast<long>(t->total_rows() - 1) * dom.nx);
-            graph.pushData(t);
+            tiles[static_cast<size_t>(i)] = std::move(t);
         }
+        const auto tb1 = clk::now();
+        for (auto& t : tiles) graph.pushData(t);
+        tiles.clear();
+        const auto tp1 = clk::now();
+        double wait_us = 0.0, stitch_us = 0.0;
         for (int got = 0; got < ntiles; ++got) {
+            const auto tw0 = clk::now();
             auto r = graph.getBlockingResult();
+            const auto tw1 = clk::now();
+            wait_us += us(tw0, tw1);
             if (!r) break;
             auto t = std::get<std::shared_ptr<HaloTile>>(*r);
             std::copy(t->data.begin() + dom.nx,
                       t->data.begin() + static_cast<long>(t->owned_rows() + 1) * dom.nx,
                       T.begin() + static_cast<long>(t->y0) * dom.nx);
+            stitch_us += us(tw1, clk::now());
         }
+        // the laser law is precomputed per step (plan.hpp); advancing is
+        // taking the post-advance position and the cruise gate of this step
+        const auto ta0 = clk::now();
+        const bool cruise = !tk.wrapped && tk.x_laser_csv >= 0.45 * dom.Lx &&
+                            tk.x_laser_csv <= 0.85 * dom.Lx;
+        const auto ta1 = clk::now();
         const Metrics m = measure(T, dom, mat.Tm);
         peak_T = std::max(peak_T, m.peak_T);
         peak_W = std::max(peak_W, m.W_um);
-        const bool cruise = !tk.wrapped && tk.x_laser_csv >= 0.45 * dom.Lx &&
-                            tk.x_laser_csv <= 0.85 * dom.Lx;
         if (cruise && m.W_sub_um > steady) steady = m.W_sub_um;
+        const auto tm1 = clk::now();
         if (tk.dump) {
             std::ofstream bin(dir + "/" + [&] {
                 char b[32];
@@ -204,17 +233,30 @@ int main(int argc, char** argv) {
             csv << s << ',' << tk.t * 1e6 << ',' << tk.x_laser_csv * 1e6 << ',' << m.peak_T << ','
                 << m.W_um << '\n';
         }
+        const auto td1 = clk::now();
+        const double step_us[kPhases] = {us(tb0, tb1), us(tb1, tp1), wait_us, stitch_us,
+                                         us(ta0, ta1), us(ta1, tm1), us(tm1, td1)};
+        perstep << s;
+        for (int q = 0; q < kPhases; ++q) {
+            perstep << ',' << step_us[q];
+            phase_us[q] += step_us[q];
+        }
+        perstep << ',' << us(tb0, td1) << '\n';
     }
     graph.finishPushingData();
     graph.waitForTermination();
     const auto wall_ms = std::chrono::duration_cast<std::chrono::milliseconds>(
-                             std::chrono::steady_clock::now() - wall0).count();
+                             clk::now() - wall0).count();
 
+    // meta.json with the reference driver's keys (src/main.cpp:428-467)
     std::ofstream meta(dir + "/meta.json");
     meta << "{\n"
          << "  \"case\":     \"" << o.case_id << "\",\n"
          << "  \"physics\":  \"" << o.physics << "\",\n"
          << "  \"kmult\":    " << o.kmult << ",\n"
+         << "  \"eta\":      " << mat.eta << ",\n"
+         << "  \"dsub_m\":   " << pr.dSub << ",\n"
+         << "  \"hconv\":    " << pr.hConv << ",\n"
          << "  \"P_W\":      " << pr.P << ",\n"
          << "  \"V_mps\":    " << pr.V << ",\n"
          << "  \"r0_m\":     " << pr.r0 << ",\n"
@@ -231,8 +273,16 @@ int main(int argc, char** argv) {
          << "  \"peak_W_um\":" << peak_W << ",\n"
          << "  \"steady_W_sub_um\": " << steady << ",\n"
          << "  \"wall_ms\":  " << wall_ms << ",\n"
+         << "  \"phases_ms\": {\n";
+    static const char* const names[kPhases] = {"build", "push", "wait", "stitch", "advance",
+                                               "metrics", "snapshot"};
+    for (int q = 0; q < kPhases; ++q)
+        meta << "    \"" << names[q] << "\": " << phase_us[q] / 1e3
+             << (q + 1 < kPhases ? ",\n" : "\n");
+    meta << "  },\n"
          << "  \"task_ms\": {\n"
          << "    \"compute_total_cpu\":  " << prof->compute_ns.load() / 1e6 << ",\n"
+         << "    \"alloc_total_cpu\":    " << prof->alloc_ns.load() / 1e6 << ",\n"
          << "    \"stencil_total_cpu\":  " << prof->stencil_ns.load() / 1e6 << ",\n"
          << "    \"n_calls\":             " << prof->n_calls.load() << "\n"
          << "  }\n"

@@ -13,24 +13,13 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import LPBF_BC_MIRROR, LaserTick, LpbfError, StepConst, check, lib
+from ._lib import (LPBF_BC_MIRROR, LaserTick, LpbfError, MrConfig, MrCounters,  # noqa: F401
+                   MrWindow, StepConst, check, lib)
 
 WIN = 128
 CH = 32
 NPARAM = 38465
 STEPS = 16
-
-
-class SgWindow(C.Structure):
-    """include/lpbf_surrogate.h lpbf_sg_window."""
-    _fields_ = [("theta", C.c_void_p), ("theta_out", C.c_void_p), ("plan", C.c_void_p),
-                ("T_ref", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
-                ("inv_r0_sq", C.c_double), ("P", C.c_double), ("V", C.c_double),
-                ("ld", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("ox", C.c_int),
-                ("oy", C.c_int), ("step", C.c_int)]
-
-
-assert C.sizeof(SgWindow) == 96
 
 
 def init_params(seed: int = 0) -> np.ndarray:
@@ -46,13 +35,6 @@ def init_params(seed: int = 0) -> np.ndarray:
     p = np.concatenate(out).astype(np.float32)
     assert p.size == NPARAM
     return p
-
-
-def window_origin(x_l, y_l, dx, dy, nx, ny):
-    """Window of WIN x WIN cells centred on the laser, clamped to the plate."""
-    ox = int(round(x_l / dx)) - WIN // 2
-    oy = int(round(y_l / dy)) - WIN // 2
-    return min(max(ox, 0), nx - WIN), min(max(oy, 0), ny - WIN)
 
 
 class Surrogate:
@@ -118,30 +100,58 @@ class Surrogate:
         return self.loss
 
 
-def windows_buffer(wins: list[SgWindow], device="cuda"):
-    import torch
-    raw = bytes((SgWindow * len(wins))(*wins))
-    return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
-
-
-def gather(win_dev, n, x0, T0):
-    lib().lpbf_sg_gather(win_dev.data_ptr(), n, x0.data_ptr(), T0.data_ptr(),
-                         _cur_stream())
-
-
-def target(win_dev, n, T0, tgt):
-    lib().lpbf_sg_target(win_dev.data_ptr(), n, T0.data_ptr(), tgt.data_ptr(), _cur_stream())
-
-
-def apply(win_dev, n, T0, y, overlap=8, skip=None, maxdiff=None):
-    lib().lpbf_sg_apply(win_dev.data_ptr(), n, T0.data_ptr(), y.data_ptr(), overlap,
-                        skip.data_ptr() if skip is not None else None,
-                        maxdiff.data_ptr() if maxdiff is not None else None, _cur_stream())
-
-
 def _cur_stream():
     import torch
     return torch.cuda.current_stream().cuda_stream
+
+
+# ---- the two-level run's grids (include/lpbf_multirate.h) ----------------------------
+
+def coarse_const(k: StepConst) -> StepConst:
+    """The coarse grid's constants, as lpbf_mr_create derives them: 2x2 cells,
+    four fine steps per coarse step, the same physics."""
+    c = StepConst.from_buffer_copy(bytes(k))
+    c.nx, c.ny = k.nx // 2, k.ny // 2
+    c.dx, c.dy = 2.0 * k.dx, 2.0 * k.dy
+    c.inv_dx2, c.inv_dy2 = 1.0 / (c.dx * c.dx), 1.0 / (c.dy * c.dy)
+    c.dt = 4.0 * k.dt
+    return c
+
+
+def coarse_plan(plan: np.ndarray, dx: float, dy: float) -> np.ndarray:
+    """One coarse tick per four fine ticks, as lpbf_mr_create builds them: the
+    mean beam strength at its strength-weighted mean position, shifted by half
+    a fine cell (coarse cell u is centred on fine position 2u + 1/2 while the
+    step kernel places it at 2u)."""
+    n4 = plan.shape[0] // 4
+    p = plan[:4 * n4].reshape(n4, 4, 3)
+    q = p[..., 2]
+    qs = q.sum(axis=1)
+    safe = np.where(qs > 0, qs, 1.0)
+    x = np.where(qs > 0, (q * p[..., 0]).sum(axis=1) / safe, p[..., 0].mean(axis=1))
+    y = np.where(qs > 0, (q * p[..., 1]).sum(axis=1) / safe, p[..., 1].mean(axis=1))
+    return np.stack([x - 0.5 * dx, y - 0.5 * dy, 0.25 * qs], axis=1)
+
+
+def window_origins(plan: np.ndarray, n_blocks: int, nx: int, ny: int, dx: float, dy: float):
+    """Window origins of blocks 0..n_blocks (lpbf_mr_window_origins: the laser
+    at 3/4 of the window along its travel, centred across it, even, inside
+    the plate)."""
+    p = np.ascontiguousarray(plan[:STEPS * n_blocks + 1], dtype=np.float64)
+    ox = np.zeros(n_blocks + 1, dtype=np.int32)
+    oy = np.zeros(n_blocks + 1, dtype=np.int32)
+    rc = lib().lpbf_mr_window_origins(p.ctypes.data, n_blocks, nx, ny, dx, dy, ox.ctypes.data,
+                                      oy.ctypes.data)
+    if rc:
+        check()
+        raise LpbfError(f"lpbf_mr_window_origins failed: {rc}")
+    return ox, oy
+
+
+def _mr_windows(wins, device="cuda"):
+    import torch
+    raw = bytes((MrWindow * len(wins))(*wins))
+    return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
 
 
 # ---- training data from FD runs ------------------------------------------------------
@@ -189,58 +199,74 @@ def make_runs(spec: DataSpec):
 
 
 def generate(spec: DataSpec, device="cuda", log=None):
-    """Runs the FD solver (FP32 batch kernel) on spec.n_runs plates and returns
-    the pairs: x (bf16 [N,128,128,4]), T0 (FP32 theta at t), target (FP32
-    (T_{t+16}-T_t)/3000); windows centred on the laser at t."""
+    """Training pairs of the two-level run's network from fine FP32 FD runs
+    (the batch kernel) and their coarse companions: every 16 steps, each
+    run's fine field is restricted onto its coarse grid, the coarse grid takes
+    its four steps, and the window centred on the laser yields the network
+    input (lpbf_mr_gather) and, after the fine run's 16 steps, the target
+    (fine increment - prolonged coarse increment) / 100 K (lpbf_mr_target).
+    Returns x (bf16 [N,128,128,4]), D (FP32 [N,128,128], prolonged coarse
+    increment, K) and the target (FP32 [N,128,128])."""
     import torch
 
     from .batch import F32Batch, InstanceSpec
+    if spec.nx % 2 or spec.ny % 2:
+        raise ValueError("the coarse grid needs even plate sides")
     runs = make_runs(spec)
     specs = [InstanceSpec(k=k, plan=plan, T0=300.0, T_ref=300.0, Tm=1609.0, size_class=0,
                           stats=False) for k, plan, _, _ in runs]
+    cspecs = [InstanceSpec(k=coarse_const(k), plan=coarse_plan(plan, k.dx, k.dy), T0=300.0,
+                           T_ref=300.0, Tm=1609.0, size_class=0, stats=False)
+              for k, plan, _, _ in runs]
     b = F32Batch(specs, bc=LPBF_BC_MIRROR, device=device)
+    cb = F32Batch(cspecs, bc=LPBF_BC_MIRROR, device=device)
     b.set_tiling(0, 4, 0)
+    cb.set_tiling(0, 2, 0)            # 4 coarse steps: two launches, the field stays put
+    n = spec.n_runs
+    nxc, nyc = spec.nx // 2, spec.ny // 2
+    ldc = cb.theta(0).shape[1]
+    c0 = torch.empty((n, nyc, ldc), dtype=torch.float32, device=device)
     n_blocks = spec.steps // STEPS - 1
-    N = n_blocks * spec.n_runs
+    N = n_blocks * n
     xs = torch.empty((N, WIN, WIN, 4), dtype=torch.bfloat16, device=device)
-    T0s = torch.empty((N, WIN, WIN), dtype=torch.float32, device=device)
+    Ds = torch.empty((N, WIN, WIN), dtype=torch.float32, device=device)
     tg = torch.empty((N, WIN, WIN), dtype=torch.float32, device=device)
-    x0 = torch.empty((spec.n_runs, WIN, WIN, CH), dtype=torch.bfloat16, device=device)
-    T0 = torch.empty((spec.n_runs, WIN, WIN), dtype=torch.float32, device=device)
+    x0 = torch.empty((n, WIN, WIN, CH), dtype=torch.bfloat16, device=device)
+    T0 = torch.empty((n, WIN, WIN), dtype=torch.float32, device=device)
+    plans = [b._bufs[i][2] for i in range(n)]
+    origins = [window_origins(plan, n_blocks, k.nx, k.ny, k.dx, k.dy) for k, plan, _, _ in runs]
+    st = _cur_stream()
     t0 = time.perf_counter()
     for blk in range(n_blocks):
         s = blk * STEPS
         wins = []
-        for i, (k, plan, P, V) in enumerate(runs):
-            xl, yl, q = plan[s]
-            ox, oy = window_origin(xl, yl, k.dx, k.dy, k.nx, k.ny)
+        for i, (k, plan, _, _) in enumerate(runs):
+            th, cth = b.theta(i), cb.theta(i)
+            lib().lpbf_mr_restrict(th.data_ptr(), th.shape[1], k.nx, k.ny, cth.data_ptr(), ldc, st)
+            lib().lpbf_mr_restrict(th.data_ptr(), th.shape[1], k.nx, k.ny, c0[i].data_ptr(), ldc,
+                                   st)
+        cb.enqueue(4 * blk, 4)
+        for i, (k, plan, _, _) in enumerate(runs):
             th = b.theta(i)
-            wins.append(SgWindow(th.data_ptr(), th.data_ptr(), b._bufs[i][2].data_ptr(), 300.0,
-                                 k.dx, k.dy, k.inv_r0_sq, P, V, th.shape[1], k.nx, k.ny, ox, oy,
-                                 s))
-        wd = windows_buffer(wins, device)
-        gather(wd, len(wins), x0, T0)
-        b.enqueue(s, STEPS)
-        # the fields moved to the other buffer: point the windows at the new state
-        wins2 = []
-        for i, w in enumerate(wins):
-            th = b.theta(i)
-            w2 = SgWindow.from_buffer_copy(bytes(w))
-            w2.theta = th.data_ptr()
-            w2.theta_out = th.data_ptr()
-            wins2.append(w2)
-        wd2 = windows_buffer(wins2, device)
-        sl = slice(blk * spec.n_runs, (blk + 1) * spec.n_runs)
-        target(wd2, len(wins2), T0, tg[sl])
+            ox, oy = int(origins[i][0][blk]), int(origins[i][1][blk])
+            wins.append(MrWindow(th.data_ptr(), c0[i].data_ptr(), cb.theta(i).data_ptr(),
+                                 plans[i].data_ptr(), k.dx, k.dy, k.inv_r0_sq, th.shape[1], ox,
+                                 oy, ox, oy, ldc, nxc, nyc, s))
+        wd = _mr_windows(wins, device)
+        sl = slice(blk * n, (blk + 1) * n)
+        lib().lpbf_mr_gather(wd.data_ptr(), n, x0.data_ptr(), Ds[sl].data_ptr(), T0.data_ptr(), st)
+        b.enqueue(s, STEPS)           # four launches: the field is back in the same buffer
+        lib().lpbf_mr_target(wd.data_ptr(), n, T0.data_ptr(), Ds[sl].data_ptr(), tg[sl].data_ptr(),
+                             st)
         xs[sl] = x0[..., :4]
-        T0s[sl] = T0
     torch.cuda.synchronize()
     check()
     if log:
         log(f"[surrogate] {N} pairs from {spec.n_runs} FD runs of {spec.steps} steps in "
             f"{time.perf_counter() - t0:.1f} s")
     b.close()
-    return xs, T0s, tg
+    cb.close()
+    return xs, Ds, tg
 
 
 def expand(x4, idx, out):
@@ -251,7 +277,10 @@ def expand(x4, idx, out):
 
 
 def train(model: Surrogate, xs, tg, iters: int, batch: int = 256, lr: float = 1e-3,
-          seed: int = 0, log=None, log_every: int = 100, time_budget: float | None = None):
+          seed: int = 0, log=None, log_every: int = 100, time_budget: float | None = None,
+          lr_final: float | None = None):
+    """Adam on random batches; with lr_final the learning rate decays
+    geometrically from lr to lr_final over the iterations."""
     import torch
     N = xs.shape[0]
     g = torch.Generator(device="cpu").manual_seed(seed)
@@ -267,12 +296,13 @@ def train(model: Surrogate, xs, tg, iters: int, batch: int = 256, lr: float = 1e
         idx = perm[pos:pos + batch].to(xs.device)
         pos += batch
         expand(xs, idx, x0)
-        loss = model.train_step(x0, tg[idx].contiguous(), lr)
+        lr_it = lr if lr_final is None else lr * (lr_final / lr) ** (it / max(iters - 1, 1))
+        loss = model.train_step(x0, tg[idx].contiguous(), lr_it)
         if (it + 1) % log_every == 0 or it == iters - 1:
             lv = float(loss.item())
             losses.append((it + 1, lv))
             if log:
-                log(f"[surrogate] iter {it + 1}: loss {lv:.3e} "
+                log(f"[surrogate] iter {it + 1}: loss {lv:.3e} lr {lr_it:.2e} "
                     f"({(time.perf_counter() - t0) / (it + 1) * 1e3:.2f} ms/iter)")
         if time_budget is not None and time.perf_counter() - t0 > time_budget:
             break
@@ -281,13 +311,17 @@ def train(model: Surrogate, xs, tg, iters: int, batch: int = 256, lr: float = 1e
     return losses
 
 
-def evaluate(model: Surrogate, xs, T0s, tg, idx, batch: int = 64):
-    """Errors in kelvin of the trained network and of persistence (T_{t+16} =
-    T_t, the method without the network) on windows idx: (rms, max) each."""
+def evaluate(model: Surrogate, xs, tg, idx, batch: int = 64, overlap: int = 8):
+    """Errors in kelvin on windows idx, over the cells at least `overlap` from
+    the window edge (where the run applies the network at full weight): the
+    trained network's residual error and that of the two-level step without
+    the network (the prolonged coarse increment alone, residual 0): rms and
+    max each."""
     import torch
     x0 = torch.empty((batch, WIN, WIN, CH), dtype=torch.bfloat16, device=xs.device)
-    se_n = se_p = 0.0
-    mx_n = mx_p = 0.0
+    inner = (slice(None), slice(overlap, WIN - overlap), slice(overlap, WIN - overlap))
+    se_n = se_b = 0.0
+    mx_n = mx_b = 0.0
     cnt = 0
     for s in range(0, len(idx), batch):
         j = idx[s:s + batch]
@@ -296,18 +330,27 @@ def evaluate(model: Surrogate, xs, T0s, tg, idx, batch: int = 64):
         jd = torch.as_tensor(j, device=xs.device)
         expand(xs, jd, x0)
         y = model.forward(x0)
-        e_n = (y - tg[jd]) * 3000.0
-        e_p = tg[jd] * 3000.0
+        e_n = ((y - tg[jd]) * 100.0)[inner]
+        e_b = (tg[jd] * 100.0)[inner]
         se_n += float((e_n.double() ** 2).sum())
-        se_p += float((e_p.double() ** 2).sum())
+        se_b += float((e_b.double() ** 2).sum())
         mx_n = max(mx_n, float(e_n.abs().max()))
-        mx_p = max(mx_p, float(e_p.abs().max()))
+        mx_b = max(mx_b, float(e_b.abs().max()))
         cnt += e_n.numel()
     return {"rms_net": math.sqrt(se_n / cnt), "max_net": mx_n,
-            "rms_persistence": math.sqrt(se_p / cnt), "max_persistence": mx_p, "windows": cnt // WIN // WIN}
+            "rms_without_net": math.sqrt(se_b / cnt), "max_without_net": mx_b,
+            "windows": cnt // (WIN - 2 * overlap) ** 2}
 
 
-# ---- surrogate-accelerated run ----------------------------------------------------------
+def save_params(model: Surrogate, path: str):
+    np.save(path, model.params.cpu().numpy())
+
+
+def load_params(path: str, device="cuda") -> Surrogate:
+    return Surrogate(params=np.load(path, allow_pickle=False), device=device)
+
+
+# ---- runs: plain FP32, FP64 reference, two-level ----------------------------------------
 
 @dataclass
 class RunSpec:
@@ -321,8 +364,6 @@ class RunSpec:
     r0: float = 50e-6
     n_tracks: int = 10
     hatch: float = 80e-6
-    depth: int = 4
-    rows: int = 0
     recheck_every: int = 8
     tol_K: float = 2.0
     overlap: int = 8
@@ -339,119 +380,149 @@ def run_setup(rs: RunSpec):
     return mat, pr, dom, make_const(mat, pr, dom, rs.dt), plan
 
 
-class _F32Field:
-    """Two pitched FP32 theta buffers, the plan on the device and the step
-    workspace (theta = T - 300 K, field starts at 300 K)."""
-
-    def __init__(self, k, plan, depth, rows, device="cuda"):
-        import torch
-        self.k, self.depth, self.rows = k, depth, rows
-        self.ld = (k.nx + 3) // 4 * 4
-        self.a = torch.zeros((k.ny, self.ld), dtype=torch.float32, device=device)
-        self.b = torch.zeros_like(self.a)
-        self.plan = torch.from_numpy(np.ascontiguousarray(plan)).to(device)
-        self.work = torch.empty(lib().lpbf_f32_workspace_bytes(k.nx, k.ny, depth, rows) // 8 + 8,
-                                dtype=torch.float64, device=device)
-
-    def advance(self, s0, n, stream):
-        s = s0
-        while s < s0 + n:
-            d = min(self.depth, s0 + n - s)
-            lib().lpbf_launch_step_f32(self.a.data_ptr(), self.b.data_ptr(), self.ld, self.k,
-                                       300.0, 0, 300.0, 1609.0, self.plan.data_ptr(), s, d,
-                                       self.rows, None, self.work.data_ptr(), stream)
-            self.a, self.b = self.b, self.a
-            s += d
-
-    def T(self):
-        return self.a[:, :self.k.nx].double() + 300.0
+def plain_run(rs: RunSpec, log=None):
+    """The plain FP32 FD run of the whole plate (the FP32 step kernel with the
+    autotuner's depth and band height, stats off).  Returns (T as FP64 on the
+    device, wall seconds)."""
+    import torch
+    mat, pr, dom, k, plan = run_setup(rs)
+    st = _cur_stream()
+    pdev = torch.from_numpy(np.ascontiguousarray(plan)).cuda()
+    bd, br, bms = C.c_int(), C.c_int(), C.c_double()
+    rc = lib().lpbf_f32_autotune(k, 300.0, LPBF_BC_MIRROR, 300.0, 1609.0, pdev.data_ptr(),
+                                 plan.shape[0], 4, st, C.byref(bd), C.byref(br), C.byref(bms))
+    if rc:
+        check()
+        raise LpbfError(f"autotune failed: {rc}")
+    depth, rows = bd.value, br.value
+    ld = (k.nx + 3) // 4 * 4
+    a = torch.zeros((k.ny, ld), dtype=torch.float32, device="cuda")
+    b = torch.zeros_like(a)
+    work = torch.empty(lib().lpbf_f32_workspace_bytes(k.nx, k.ny, depth, rows) // 8 + 8,
+                       dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s = 0
+    while s < rs.steps:
+        d = min(depth, rs.steps - s)
+        lib().lpbf_launch_step_f32(a.data_ptr(), b.data_ptr(), ld, k, 300.0, LPBF_BC_MIRROR,
+                                   300.0, 1609.0, pdev.data_ptr(), s, d, rows, None,
+                                   work.data_ptr(), st)
+        a, b = b, a
+        s += d
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    check()
+    if log:
+        log(f"[plain] {rs.nx}x{rs.ny} {rs.steps} steps: {wall:.2f} s (depth {depth}, "
+            f"rows {rows})")
+    return a[:, :k.nx].double() + 300.0, wall
 
 
 def fp64_reference(k, plan, steps, device="cuda"):
-    """The FP64 parity path (lpbf_launch_step_ex, mirror edges) over the run;
-    its step equals the CPU oracle's to 1e-9 K (tests/test_gpu_f64.py)."""
+    """The FP64 parity path over the run (the temporally blocked FP64 kernel at
+    depth 2, bitwise the single-step field, which equals the CPU oracle's to
+    1e-9 K), mirror edges."""
     import torch
-
-    from ._lib import StepConst as SC
     a = torch.full((k.ny, k.nx), 300.0, dtype=torch.float64, device=device)
     b = torch.empty_like(a)
-    kk = SC.from_buffer_copy(bytes(k))
-    st = torch.cuda.current_stream().cuda_stream
-    for s in range(steps):
-        kk.x_l, kk.y_l, kk.q_scale = plan[s]
-        lib().lpbf_launch_step_ex(a.data_ptr(), b.data_ptr(), kk, 0, 0, 300.0, 1609.0, None, st)
+    pdev = torch.from_numpy(np.ascontiguousarray(plan)).to(device)
+    st = _cur_stream()
+    s = 0
+    while s < steps:
+        d = min(2, steps - s)
+        lib().lpbf_launch_step_f64_blocked(a.data_ptr(), b.data_ptr(), k, 0, LPBF_BC_MIRROR,
+                                           300.0, 1609.0, pdev.data_ptr(), s, d, 0, None, None,
+                                           st)
         a, b = b, a
+        s += d
     torch.cuda.synchronize()
     check()
     return a
 
 
-def accelerated_run(model: Surrogate | None, rs: RunSpec, log=None):
-    """Plain FP32 FD over the whole plate, 16 steps at a time; with a model,
-    after every 16 steps the 128 x 128 window centred on the laser at the start
-    of the block is replaced by the surrogate's prediction from the field at
-    the start (blended over rs.overlap border cells).  Every rs.recheck_every
-    calls the prediction is compared with the FD result of the same window; a
-    difference above rs.tol_K keeps the FD result (fallback) and the next
-    rs.recheck_every - 1 calls are also left to FD.  Returns the final FP32
-    field (as T), wall seconds and counters."""
+class TwoLevelRun:
+    """lpbf_mr_* run object: coarse far field plus the fine melt-pool window,
+    advanced by `model` (with FD rechecks and fallback) or, without a model,
+    by the FD patch every block."""
+
+    def __init__(self, rs: RunSpec, model: Surrogate | None = None, tol_K: float | None = None,
+                 recheck_every: int | None = None):
+        import torch
+        self.torch = torch
+        self.rs = rs
+        self.mat, self.pr, self.dom, self.k, self.plan = run_setup(rs)
+        cfg = MrConfig()
+        cfg.k = self.k
+        cfg.T_ref = 300.0
+        cfg.tol_K = rs.tol_K if tol_K is None else tol_K
+        cfg.n_steps = rs.steps
+        cfg.recheck_every = rs.recheck_every if recheck_every is None else recheck_every
+        cfg.overlap = rs.overlap
+        cfg.use_network = int(model is not None)
+        cfg.coarse_depth = 0                  # autotuned
+        cfg.coarse_rows = 0
+        self.model = model
+        plan = np.ascontiguousarray(self.plan, dtype=np.float64)
+        h = C.c_void_p()
+        rc = lib().lpbf_mr_create(C.byref(cfg), plan.ctypes.data,
+                                  model.pack.data_ptr() if model is not None else None,
+                                  _cur_stream(), C.byref(h))
+        if rc:
+            check()
+            raise LpbfError(f"lpbf_mr_create failed: {rc}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lpbf_mr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def advance(self, n_blocks: int):
+        rc = lib().lpbf_mr_advance(self._h, n_blocks)
+        if rc:
+            check()
+            raise LpbfError(f"lpbf_mr_advance failed: {rc}")
+
+    def field(self):
+        T = self.torch.empty((self.k.ny, self.k.nx), dtype=self.torch.float64, device="cuda")
+        rc = lib().lpbf_mr_field(self._h, T.data_ptr())
+        if rc:
+            raise LpbfError(f"lpbf_mr_field failed: {rc}")
+        return T
+
+    def counters(self) -> dict:
+        c = MrCounters()
+        lib().lpbf_mr_get_counters(self._h, C.byref(c))
+        return {f: getattr(c, f) for f, _ in MrCounters._fields_}
+
+    def recheck_diffs(self):
+        n = lib().lpbf_mr_recheck_diffs(self._h, None, 0)
+        out = np.zeros(max(n, 1), dtype=np.float32)
+        lib().lpbf_mr_recheck_diffs(self._h, out.ctypes.data, n)
+        return out[:n]
+
+
+def two_level_run(rs: RunSpec, model: Surrogate | None = None, log=None, **kw):
+    """Whole run timed: returns (T as FP64 on the device, wall seconds,
+    counters, recheck differences)."""
     import torch
-    mat, pr, dom, k, plan = run_setup(rs)
-    f = _F32Field(k, plan, rs.depth, rs.rows)
-    stream = torch.cuda.current_stream().cuda_stream
-    x0 = torch.empty((1, WIN, WIN, CH), dtype=torch.bfloat16, device="cuda")
-    T0 = torch.empty((1, WIN, WIN), dtype=torch.float32, device="cuda")
-    y = torch.empty((1, WIN, WIN), dtype=torch.float32, device="cuda")
-    md = torch.zeros(1, dtype=torch.float32, device="cuda")
-    skip = torch.zeros(1, dtype=torch.int32, device="cuda")
-    calls = rechecks = fallbacks = applied = 0
-    off_until = -1
-    since_check = 0     # calls applied since the last recheck; 0: the next call is checked
-    diffs = []
+    r = TwoLevelRun(rs, model, **kw)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    n_blocks = rs.steps // STEPS
-    for blk in range(n_blocks):
-        s = blk * STEPS
-        use = model is not None and blk > off_until and plan[s][2] != 0.0
-        if use:
-            ox, oy = window_origin(plan[s][0], plan[s][1], k.dx, k.dy, k.nx, k.ny)
-            w = SgWindow(f.a.data_ptr(), 0, f.plan.data_ptr(), 300.0, k.dx, k.dy, k.inv_r0_sq,
-                         rs.P, rs.V, f.ld, k.nx, k.ny, ox, oy, s)
-            wd = windows_buffer([w])
-            gather(wd, 1, x0, T0)
-            model.forward(x0, out=y)
-        f.advance(s, STEPS, stream)
-        if use:
-            calls += 1
-            w.theta_out = f.a.data_ptr()
-            wd = windows_buffer([w])
-            if since_check == 0:
-                # compare with the FD result of the same window, write nothing
-                md.zero_()
-                skip.fill_(1)
-                apply(wd, 1, T0, y, rs.overlap, skip, md)
-                d = float(md.item())
-                diffs.append(d)
-                rechecks += 1
-                if d > rs.tol_K:
-                    # keep FD here and for the next recheck_every - 1 blocks;
-                    # the first call after that is checked again
-                    fallbacks += 1
-                    off_until = blk + rs.recheck_every - 1
-                    continue
-            apply(wd, 1, T0, y, rs.overlap, None, None)
-            applied += 1
-            since_check = (since_check + 1) % rs.recheck_every
-    f.advance(n_blocks * STEPS, rs.steps - n_blocks * STEPS, stream)
+    r.advance(rs.steps // STEPS)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     check()
-    out = {"wall_s": wall, "sim_s": rs.steps * rs.dt, "calls": calls, "applied": applied,
-           "rechecks": rechecks, "fallbacks": fallbacks,
-           "fallback_rate": fallbacks / rechecks if rechecks else None,
-           "recheck_maxdiff_K": diffs}
+    T = r.field()
+    torch.cuda.synchronize()
+    cnt, diffs = r.counters(), r.recheck_diffs()
+    r.close()
     if log:
-        log(f"[surrogate-run] {'model' if model is not None else 'plain'}: {wall:.2f} s wall, "
-            f"{calls} calls, {applied} applied, {fallbacks}/{rechecks} rechecks fell back")
-    return f.T(), out
+        log(f"[two-level] {'network' if model is not None else 'FD window'}: {wall:.2f} s, "
+            f"{cnt['calls']} calls, {cnt['applied']} applied, {cnt['fallbacks']}/"
+            f"{cnt['rechecks']} rechecks fell back, {cnt['fd_blocks']} FD blocks")
+    return T, wall, cnt, diffs

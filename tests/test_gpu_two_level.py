@@ -1,0 +1,201 @@
+"""The two-level run (include/lpbf_multirate.h): transfer operators, the FD
+window, the fallback path and the learned window against the FP64 path.
+
+Pins:
+  * lpbf_mr_restrict is the 2x2 mean (bitwise, same operation order);
+  * the prolongation in lpbf_mr_gather reproduces an affine coarse increment
+    exactly at the fine cell centres away from the plate edge (so its weights
+    and quarter-cell offsets are right), the fine-scale channel vanishes for
+    a field that is constant over the window, and the source map follows
+    its closed form;
+  * lpbf_mr_target is (theta1 - T0 - D) / 100;
+  * the FD window after one block from a uniform start (no coarse error
+    anywhere) equals the plain FP32 plate step over the window;
+  * with tolerance 0 every recheck falls back, and the run is bitwise the
+    FD-window run;
+  * the FD-window run and the learned run stay within a stated bound of the
+    FP64 path on a reduced plate (1024^2, two tracks, 4096 steps); with the
+    shipped network and a recheck on every call, some windows are applied
+    and every applied window is within 2 K of its FD recheck.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import hedgehog_lpbf_amd as h
+from hedgehog_lpbf_amd import surrogate as sg
+from hedgehog_lpbf_amd._lib import MrWindow
+from tests.devutil import sync_check
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = os.path.join(os.path.dirname(sg.__file__), "data", "surrogate_params.npy")
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def test_restrict_is_the_2x2_mean():
+    print("\n[case] lpbf_mr_restrict 300x200 plate", flush=True)
+    rng = np.random.default_rng(1)
+    nx, ny, ld, ldc = 300, 200, 304, 152
+    f = rng.uniform(-5, 3000, (ny, ld)).astype(np.float32)
+    d = torch.from_numpy(f).cuda()
+    c = torch.zeros((ny // 2, ldc), dtype=torch.float32, device="cuda")
+    h.lib().lpbf_mr_restrict(d.data_ptr(), ld, nx, ny, c.data_ptr(), ldc, _stream())
+    sync_check()
+    a = f[:, :nx]
+    want = np.float32(0.25) * ((a[0::2, 0::2] + a[0::2, 1::2]) + (a[1::2, 0::2] + a[1::2, 1::2]))
+    assert np.array_equal(c.cpu().numpy()[:, :nx // 2], want)
+
+
+def _window(fine, ldf, fx, fy, ox, oy, c0, c1, ldc, nxc, nyc, plan, step, dx=5e-6, r0=50e-6):
+    return MrWindow(fine.data_ptr(), c0.data_ptr(), c1.data_ptr(), plan.data_ptr(), dx, dx,
+                    1.0 / r0 ** 2, ldf, fx, fy, ox, oy, ldc, nxc, nyc, step)
+
+
+def test_gather_and_target_against_closed_forms():
+    print("\n[case] lpbf_mr_gather / lpbf_mr_target, two windows", flush=True)
+    nx = ny = 512
+    nxc, nyc, ldc = 256, 256, 256
+    dx = 5e-6
+    u = np.arange(nxc, dtype=np.float64)[None, :]
+    v = np.arange(nyc, dtype=np.float64)[:, None]
+    c0 = np.full((nyc, ldc), 100.0, dtype=np.float32)
+    inc = 2.0 + 0.75 * u + 0.5 * v                      # affine coarse increment
+    c1 = (c0 + inc).astype(np.float32)
+    fine = np.full((ny, nx), 1234.0, dtype=np.float32)  # constant field
+    rng = np.random.default_rng(2)
+    fine1 = (fine + rng.normal(0, 30, (ny, nx))).astype(np.float32)
+    plan = np.zeros((64, 3))
+    plan[:, 0] = (200.3 + 0.8 * np.arange(64)) * dx
+    plan[:, 1] = 250.6 * dx
+    plan[:, 2] = 1.0
+    plan[40:, 2] = 0.0
+    dev = {k: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+           for k, a in dict(c0=c0, c1=c1, fine=fine, fine1=fine1, plan=plan).items()}
+    wins = [_window(dev["fine"], nx, 100, 180, 100, 180, dev["c0"], dev["c1"], ldc, nxc, nyc,
+                    dev["plan"], 0),
+            _window(dev["fine"], nx, 240, 300, 240, 300, dev["c0"], dev["c1"], ldc, nxc, nyc,
+                    dev["plan"], 30)]
+    wd = sg._mr_windows(wins)
+    x0 = torch.empty((2, 128, 128, 32), dtype=torch.bfloat16, device="cuda")
+    D = torch.empty((2, 128, 128), dtype=torch.float32, device="cuda")
+    T0 = torch.empty_like(D)
+    h.lib().lpbf_mr_gather(wd.data_ptr(), 2, x0.data_ptr(), D.data_ptr(), T0.data_ptr(),
+                           _stream())
+    sync_check()
+    x = x0.float().cpu().numpy()
+    Dn = D.cpu().numpy()
+    yy, xx = np.mgrid[0:128, 0:128]
+    for b, w in enumerate(wins):
+        X, Y = w.ox + xx, w.oy + yy
+        # cell-centred coarse coordinates of the fine centres: (X - 1/2) / 2
+        want = 2.0 + 0.75 * (X - 0.5) / 2 + 0.5 * (Y - 0.5) / 2
+        assert np.abs(Dn[b] - want).max() <= 1e-3, b
+        assert np.array_equal(T0[b].cpu().numpy(), fine[w.oy:w.oy + 128, w.ox:w.ox + 128])
+        assert not x[b, ..., 0].any()                                   # no fine-scale part
+        th = np.float32(1234.0) * np.float32(1.0 / 3000.0)
+        assert np.all(x[b, ..., 1] == torch.tensor(float(th)).bfloat16().float().item())
+        assert np.abs(x[b, ..., 2] - want / 1000).max() <= 2 ** -8 * (want.max() / 1000)
+        src = np.zeros((128, 128))
+        for s in range(16):
+            xl, yl, q = plan[w.step + s]
+            if q:
+                src += q * np.exp(-2 * ((X * dx - xl) ** 2 + (Y * dx - yl) ** 2) / (50e-6) ** 2)
+        src /= 16
+        assert np.abs(x[b, ..., 3] - src).max() <= 2 ** -8 * max(src.max(), 1e-30) + 1e-6, b
+        assert not x[b, ..., 4:].any()
+    # the target from the field 16 steps later
+    wins1 = [MrWindow.from_buffer_copy(bytes(w)) for w in wins]
+    for w in wins1:
+        w.fine = dev["fine1"].data_ptr()
+    tg = torch.empty_like(D)
+    h.lib().lpbf_mr_target(sg._mr_windows(wins1).data_ptr(), 2, T0.data_ptr(), D.data_ptr(),
+                           tg.data_ptr(), _stream())
+    sync_check()
+    for b, w in enumerate(wins):
+        f1 = fine1[w.oy:w.oy + 128, w.ox:w.ox + 128]
+        want = ((f1 - T0[b].cpu().numpy()) - Dn[b]) * np.float32(0.01)
+        assert np.array_equal(tg[b].cpu().numpy(), want)
+
+
+def _small(steps=4096, n=1024):
+    return sg.RunSpec(nx=n, ny=n, steps=steps, n_tracks=2)
+
+
+def test_fd_window_first_block_equals_plate_step():
+    """One block from the uniform start: the coarse grid is exactly zero
+    excess everywhere the laser has not reached, so the window's FD result
+    must equal the plain FP32 step of the plate over the window."""
+    print("\n[case] two-level FD window, first block vs plain FP32 plate", flush=True)
+    rs = _small(steps=512, n=512)
+    r = sg.TwoLevelRun(rs, None)
+    r.advance(1)
+    T = r.field()
+    sync_check()
+    mat, pr, dom, k, plan = sg.run_setup(rs)
+    ox, oy = sg.window_origins(plan, 1, k.nx, k.ny, k.dx, k.dy)
+    ld = k.nx
+    a = torch.zeros((k.ny, ld), dtype=torch.float32, device="cuda")
+    b = torch.zeros_like(a)
+    pdev = torch.from_numpy(np.ascontiguousarray(plan)).cuda()
+    work = torch.empty(h.lib().lpbf_f32_workspace_bytes(k.nx, k.ny, 4, 0) // 8 + 8,
+                       dtype=torch.float64, device="cuda")
+    for s in range(0, 16, 4):
+        h.lib().lpbf_launch_step_f32(a.data_ptr(), b.data_ptr(), ld, k, 300.0, 0, 300.0, 1e9,
+                                     pdev.data_ptr(), s, 4, 0, None, work.data_ptr(), _stream())
+        a, b = b, a
+    sync_check()
+    plate = a.double() + 300.0
+    # the cells the window kept: old window (block 0) within the new one (block 1)
+    x0, y0 = max(ox[0], ox[1]), max(oy[0], oy[1])
+    x1, y1 = min(ox[0], ox[1]) + 128, min(oy[0], oy[1]) + 128
+    got = T[y0:y1, x0:x1]
+    want = plate[y0:y1, x0:x1]
+    err = float((got - want).abs().max())
+    print(f"peak {float(want.max()):.1f} K, max|FD window - plate| {err:.2e} K")
+    assert float(want.max()) > 1000.0
+    assert err <= 1e-2
+    r.close()
+
+
+def test_zero_tolerance_falls_back_to_the_fd_window_bitwise():
+    print("\n[case] two-level: tolerance 0 -> every recheck falls back", flush=True)
+    rs = _small(steps=512, n=512)
+    model = sg.Surrogate(seed=3)          # any network: every recheck fails at 0 K
+    Tf, _, cf, _ = sg.two_level_run(rs, None)
+    Tn, _, cn, d = sg.two_level_run(rs, model, tol_K=0.0, recheck_every=1)
+    nb = rs.steps // 16
+    assert cf["fd_blocks"] == nb and cf["calls"] == 0
+    assert cn["calls"] == cn["rechecks"] == cn["fallbacks"] == cn["fd_blocks"] == nb
+    assert cn["applied"] == 0 and len(d) == nb and np.all(d > 0)
+    assert torch.equal(Tf, Tn)
+
+
+@pytest.mark.skipif(not os.path.exists(PARAMS), reason="no shipped surrogate parameters")
+def test_learned_run_applies_windows_within_tolerance_and_tracks_fp64():
+    print("\n[case] two-level learned run, 1024^2, 2 tracks, 4096 steps, recheck every call",
+          flush=True)
+    rs = _small()
+    mat, pr, dom, k, plan = sg.run_setup(rs)
+    T64 = sg.fp64_reference(k, plan, rs.steps)
+    Tf, wf, cf, _ = sg.two_level_run(rs, None)
+    model = sg.load_params(PARAMS)
+    Tn, wn, cn, d = sg.two_level_run(rs, model, tol_K=2.0, recheck_every=1)
+    ef = float((Tf - T64).abs().max())
+    en = float((Tn - T64).abs().max())
+    print(f"max|dT| vs FP64: FD window {ef:.2f} K, learned {en:.2f} K; peak {float(T64.max()):.0f} K; "
+          f"{cn['applied']}/{cn['calls']} applied, recheck median {float(np.median(d)):.2f} K")
+    assert cn["calls"] == cn["rechecks"] == rs.steps // 16
+    assert cn["applied"] > 0
+    assert cn["applied"] + cn["fallbacks"] == cn["calls"]
+    assert cn["max_applied_recheck_K"] <= 2.0
+    assert np.sum(d <= 2.0) == cn["applied"]
+    # stated bounds against the FP64 path (measured 2-4 K and 2-4 K on this
+    # plate; the 4096^2 run is reported by bench.py --workload surrogate)
+    assert ef <= 10.0 and en <= 10.0
+    assert float((Tn - T64).abs().mean()) <= 0.05
