@@ -58,12 +58,16 @@ struct F32Args {
     // convection + substrate are linear in theta, radiation expands
     // (T_ref + theta)^4 - Tamb^4 around T_ref.
     float a0, a1, a2, a3, a4;
+    // the centre part of the update, kc*theta - dt*loss(theta), as one
+    // polynomial: p0 = -a0, p1 = kc - a1, p2 = -a2, p3 = -a3, p4 = -a4
+    float p0, p1, p2, p3, p4;
     float theta_m;         // Tm - T_ref
     float theta_bc;        // T_bc - T_ref
     float ax, ay;          // log2(e) * 2*dx^2/r0^2, log2(e) * 2*dy^2/r0^2
     float cut;             // base-2 exponent beyond which the source is dropped
     double inv_dx, inv_dy, cS0;  // 1/dx, 1/dy, dt*peak_S/(rho_cp*h)
     double Tref_d;
+    double kc_d;           // kc in FP64: the loss sum is kc * sum(theta) - sum(centre part)
     StepStats* part;       // [depth][nwg]
 };
 
@@ -113,7 +117,7 @@ struct LevelAcc {
     int ymin, ymax;  // molten rows of the wavefront's owned cells (wave-uniform)
     int molten;      // molten owned cells of the wavefront (wave-uniform)
     double S;        // sum of theta over the owned cells after this step
-    float SL;        // sum of dt*loss
+    float SP;        // sum of the step's centre parts kc*theta - dt*loss
     float SS;        // sum of dt*S
 };
 
@@ -179,6 +183,11 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         const bool own_lane = lane >= 1 && lane <= kWave - 2 && col4 < nx;
         owner = own_lane;
         own_mask = __builtin_amdgcn_ballot_w64(own_lane);
+        // the polynomial's leading coefficient in a VGPR: its first FMA
+        // would otherwise read two scalar operands, one more than a VALU
+        // instruction may, and take a copy on every row and level
+        float lead;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(lead) : "s"(RAD ? a.p4 : a.p1));
         const int ya = band * a.R;
         const int yb = min(ya + a.R, ny);
         const int y_lo = max(ya - K, 0);
@@ -212,13 +221,11 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                 }
             }
         };
-        // Dirichlet: columns 0 and nx-1 := theta_bc; their loss leaves the sums.
-        auto fix_dirichlet = [&](float4& v, float4& L) {
-            if (has_left && lane == 1) { v.x = a.theta_bc; L.x = 0.f; }   // column 0
-            if (has_last && lane == lastl) {
-                set_elem(v, laste, a.theta_bc);
-                set_elem(L, laste, 0.f);
-            }
+        // Dirichlet: columns 0 and nx-1 := theta_bc (dir_cols keeps them out
+        // of the sums).
+        auto fix_dirichlet = [&](float4& v) {
+            if (has_left && lane == 1) v.x = a.theta_bc;   // column 0
+            if (has_last && lane == lastl) set_elem(v, laste, a.theta_bc);
         };
         // Dirichlet edge cells are held, not updated, and stay out of the
         // energy sums: columns 0 and nx-1 drop out of a row's theta sum
@@ -314,22 +321,26 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             const float xp[4] = {c.y, c.z, c.w, wr};
             const float uv[4] = {up.x, up.y, up.z, up.w};
             const float dv[4] = {dn.x, dn.y, dn.z, dn.w};
-            // theta' = kc*theta - L + S + cy*(up + down) + cx*(left + right),
+            // theta' = P(theta) + S + cy*(up + down) + cx*(left + right) with
+            // the centre part P(theta) = kc*theta - L(theta) one polynomial,
             // the last term added after the source
-            float out4[4], loss4[4];
+            float out4[4], cen4[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const float th = cv[e];
-                float L;
-                if (RAD) L = fmaf(fmaf(fmaf(fmaf(a.a4, th, a.a3), th, a.a2), th, a.a1), th, a.a0);
-                else L = fmaf(a.a1, th, a.a0);
-                out4[e] = fmaf(a.cy, uv[e] + dv[e], fmaf(a.kc, th, -L));
-                loss4[e] = L;
+                float P;
+                if (RAD) P = fmaf(fmaf(fmaf(fmaf(lead, th, a.p3), th, a.p2), th, a.p1), th, a.p0);
+                else P = fmaf(lead, th, a.p0);
+                out4[e] = fmaf(a.cy, uv[e] + dv[e], P);
+                cen4[e] = P;
             }
-            // source: the whole row is skipped unless it is near the laser
-            float ssum = 0.f;
+            const bool own_row = r >= ya && r < yb;
+            const bool drow = GEN && BC == LPBF_BC_DIRICHLET && (r == 0 || r == ny - 1);
+            // source: the whole row is skipped unless it is near the laser,
+            // and only source rows touch the source sum
             const bool src_row = r >= srlo[j - 1] && r <= srhi[j - 1];
             if (src_row) {
+                float ssum = 0.f;
                 const float uy = static_cast<float>(r) - yc[j - 1];
                 const float ey = a.ay * uy * uy;
 #pragma unroll
@@ -345,21 +356,22 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                     if (BC == LPBF_BC_DIRICHLET) ssum += (cc >= 1 && cc < nx - 1) ? sv : 0.f;
                     else ssum += (!RAGGED || cc < nx) ? sv : 0.f;
                 }
+                // a select, not a nested branch (which would route the
+                // wave-uniform condition through a VGPR on every row)
+                acc[j - 1].SS += (own_row && !drow) ? ssum : 0.f;
+                pin(acc[j - 1].SS);
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) out4[e] = fmaf(a.cx, xm[e] + xp[e], out4[e]);
             float4 res = make_float4(out4[0], out4[1], out4[2], out4[3]);
-            float4 L4 = make_float4(loss4[0], loss4[1], loss4[2], loss4[3]);
-            const bool drow = GEN && BC == LPBF_BC_DIRICHLET && (r == 0 || r == ny - 1);
+            const float4 P4 = make_float4(cen4[0], cen4[1], cen4[2], cen4[3]);
             if (drow) {
                 res = make_float4(a.theta_bc, a.theta_bc, a.theta_bc, a.theta_bc);
-                L4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                ssum = 0.f;
             } else if (edge) {
-                if (BC == LPBF_BC_DIRICHLET) fix_dirichlet(res, L4);
+                if (BC == LPBF_BC_DIRICHLET) fix_dirichlet(res);
                 else fix_mirror(res);
             }
-            if (r >= ya && r < yb) {
+            if (own_row) {
                 if (j == K && own_lane)
                     *reinterpret_cast<float4*>((BUFS ? dst : a.out) + static_cast<long long>(r) * a.ld + col4) = res;
                 LevelAcc& A = acc[j - 1];
@@ -389,15 +401,18 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                     A.ymin = min(A.ymin, r);
                     A.ymax = r;
                 }
+                // the loss sum is kc * (sum of the input theta, the previous
+                // level's S) - (sum of the centre parts), formed in FP64 from
+                // the wavefront sums: the same cells enter both sums.  The
+                // per-lane centre sum stays FP32 (relative error ~1e-4 of a
+                // step's loss sum, from its rounding at the hot lanes)
                 A.S += drow ? 0.0 : static_cast<double>(row_sum(dir_cols(res)));
-                A.SL += row_sum(L4);
-                A.SS += ssum;   // 0 off the source rows: an add, no select
+                A.SP += drow ? 0.f : row_sum(dir_cols(P4));
                 // keep each running sum updated here: left alone the compiler
                 // defers the serial adds to the end of the unrolled trip and
                 // holds every phase's partial live (about 60 extra VGPRs)
                 pin(A.S);
-                pin(A.SL);
-                pin(A.SS);
+                pin(A.SP);
             }
             if constexpr (j < K) {
                 win_new(PHC, JC) = res;
@@ -485,7 +500,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             acc[j].bv = kNegInf;
             acc[j].bi = 0x7fffffff;
             acc[j].S = 0.0;
-            acc[j].SL = acc[j].SS = 0.f;
+            acc[j].SP = acc[j].SS = 0.f;
         }
         S0 = 0.0;
     }
@@ -501,7 +516,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         const PartialAcc w = wave_reduce(p);
         const double wS = wave_sum(A.S);
         const double ws = wave_sum(static_cast<double>(A.SS));
-        const double wl = wave_sum(static_cast<double>(A.SL));
+        const double wl = a.kc_d * prevS - wave_sum(static_cast<double>(A.SP));   // dt*loss
         const long long wm = A.molten;   // already the wavefront's count
         if (lane == 0) {
             s_pa[wave] = w;
@@ -587,6 +602,14 @@ inline F32Args make_args(const float* in, float* out, int ld, const StepConst& k
     a.a2 = static_cast<float>(6.0 * cR * Tr * Tr);
     a.a3 = static_cast<float>(4.0 * cR * Tr);
     a.a4 = static_cast<float>(cR);
+    const double kc = 1.0 - 2.0 * k.alpha * dt * (k.inv_dx2 + k.inv_dy2);
+    a.kc_d = kc;
+    a.p0 = static_cast<float>(-(cH * (Tr - Ta) + cB * (Tr - k.Tsub) +
+                                cR * (Tr * Tr * Tr * Tr - Ta * Ta * Ta * Ta)));
+    a.p1 = static_cast<float>(kc - (cH + cB + 4.0 * cR * Tr * Tr * Tr));
+    a.p2 = static_cast<float>(-6.0 * cR * Tr * Tr);
+    a.p3 = static_cast<float>(-4.0 * cR * Tr);
+    a.p4 = static_cast<float>(-cR);
     a.theta_m = static_cast<float>(Tm - T_ref);
     a.theta_bc = static_cast<float>(T_bc - T_ref);
     const double log2e = 1.4426950408889634;

@@ -160,7 +160,8 @@ __global__ __launch_bounds__(kT) void k_mr_target(const lpbf_mr_window* __restri
 // region's edge error has reached 15 cells in, so the owned cells are exact.
 // The region starts from the window's cells and, outside it, the prolonged
 // coarse field.  Every cell takes the FP32 step kernel's expression
-// (theta' = kc*theta - loss + source + cy*(up+down) + cx*(left+right)).
+// (theta' = P(theta) + source + cy*(up+down) + cx*(left+right), P the
+// centre part kc*theta - loss as one polynomial).
 constexpr int kFdTile = 16;
 constexpr int kFdReg = kFdTile + 2 * kSteps;   // 48
 constexpr int kFdGroups = kT / kFdReg;          // 5 row groups of 48 threads
@@ -214,10 +215,10 @@ __global__ __launch_bounds__(kT) void k_mr_fd_window(F32Args a, const float* __r
                 const float dn = A[min(r + 1, RH - 1) * kFdReg + c];
                 const float lf = A[r * kFdReg + cl];
                 const float rt = A[r * kFdReg + cr];
-                float L;
-                if (RAD) L = fmaf(fmaf(fmaf(fmaf(a.a4, th, a.a3), th, a.a2), th, a.a1), th, a.a0);
-                else L = fmaf(a.a1, th, a.a0);
-                float v = fmaf(a.cy, up + dn, fmaf(a.kc, th, -L));
+                float P;
+                if (RAD) P = fmaf(fmaf(fmaf(fmaf(a.p4, th, a.p3), th, a.p2), th, a.p1), th, a.p0);
+                else P = fmaf(a.p1, th, a.p0);
+                float v = fmaf(a.cy, up + dn, P);
                 const int Y = ry0 + r;
                 if (Y >= srlo && Y <= srhi) {
                     const float uy = static_cast<float>(Y) - yc;
@@ -286,10 +287,10 @@ __global__ __launch_bounds__(kCwThreads) void k_mr_coarse_window(F32Args a,
                 const float dn = A[min(r + 1, RH - 1) * kCwReg + c];
                 const float lf = A[r * kCwReg + cl];
                 const float rt = A[r * kCwReg + cr];
-                float L;
-                if (RAD) L = fmaf(fmaf(fmaf(fmaf(a.a4, th, a.a3), th, a.a2), th, a.a1), th, a.a0);
-                else L = fmaf(a.a1, th, a.a0);
-                float v = fmaf(a.cy, up + dn, fmaf(a.kc, th, -L));
+                float P;
+                if (RAD) P = fmaf(fmaf(fmaf(fmaf(a.p4, th, a.p3), th, a.p2), th, a.p1), th, a.p0);
+                else P = fmaf(a.p1, th, a.p0);
+                float v = fmaf(a.cy, up + dn, P);
                 const int Y = ry0 + r;
                 if (Y >= srlo && Y <= srhi) {
                     const float uy = static_cast<float>(Y) - yc;
