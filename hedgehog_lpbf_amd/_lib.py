@@ -187,6 +187,7 @@ SIGNATURES = {
     # two-level run (include/lpbf_multirate.h)
     "lpbf_mr_restrict": (_v, [_vp, _i, _i, _i, _vp, _i, _vp]),
     "lpbf_mr_window_origins": (_i, [_vp, _i, _i, _i, _d, _d, _vp, _vp]),
+    "lpbf_mr_coarse_coeffs": (_i, [_KP, _d, _vp]),
     "lpbf_mr_gather": (_v, [_vp, _i, _vp, _vp, _vp, _vp]),
     "lpbf_mr_target": (_v, [_vp, _i, _vp, _vp, _vp, _vp]),
     "lpbf_mr_create": (_i, [C.POINTER(MrConfig), _vp, _vp, _vp, C.POINTER(C.c_void_p)]),

@@ -48,6 +48,22 @@ inline std::vector<int> band_candidates(int ny, long long nstrips, long long slo
 // Records the first error since the last lpbf_get_last_error() query.
 void record_error(int err);
 
+// Coefficients that replace a FP32 step launch's own (derived from its
+// StepConst): the x and y diffusion numbers and the centre polynomial
+// P(theta) = p[0] + p[1] theta + ... + p[4] theta^4, i.e. kc*theta minus
+// the loss over the step.  The two-level run's coarse grid takes these so
+// that one coarse step reproduces four fine steps' loss (lpbf_multirate.hip).
+struct F32Coeffs {
+    double cx, cy;
+    double p[5];
+};
+
+// lpbf_launch_step_f32 (mirror edges, no statistics) with `c` in place of
+// the step's own coefficients; the arguments are the caller's to check.
+void step_f32_launch_coeffs(const float* theta_in, float* theta_out, int ld, const StepConst& k,
+                            double T_ref, const F32Coeffs& c, const LaserTick* d_plan, int step0,
+                            int depth, int rows, void* work, hipStream_t stream);
+
 // After a launch: keep the runtime's sticky error visible to the caller
 // (reference convention, gpu_step_task.hpp:158) and record it here too.
 inline void check_launch() {

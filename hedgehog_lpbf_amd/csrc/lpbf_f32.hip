@@ -156,6 +156,22 @@ void* scratch_of_current_device() {
 }
 
 }  // namespace
+
+void step_f32_launch_coeffs(const float* theta_in, float* theta_out, int ld, const StepConst& k,
+                            double T_ref, const F32Coeffs& c, const LaserTick* d_plan, int step0,
+                            int depth, int rows, void* work, hipStream_t stream) {
+    F32Args a = make_args(theta_in, theta_out, ld, k, T_ref, LPBF_BC_MIRROR, T_ref, T_ref + 1e9,
+                          depth, rows, work);
+    set_coeffs(a, c.cx, c.cy, c.p);
+    switch (depth) {
+        case 1: launch_k<1>(a, d_plan, step0, stream); break;
+        case 2: launch_k<2>(a, d_plan, step0, stream); break;
+        case 3: launch_k<3>(a, d_plan, step0, stream); break;
+        default: launch_k<4>(a, d_plan, step0, stream); break;
+    }
+    check_launch();
+}
+
 }  // namespace lpbf_amd
 
 using namespace lpbf_amd;

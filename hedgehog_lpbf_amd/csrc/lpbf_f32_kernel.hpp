@@ -578,6 +578,42 @@ inline bool f32_shape_ok(long long nx, long long ny) {
            nx * ny < (1LL << 31) - 1;
 }
 
+// dt*loss(theta) in K per step as a polynomial in theta = T - T_ref
+// (FP64): c[0] + c[1] theta + ... + c[4] theta^4, from
+//   cH*(theta + T_ref - Tamb) + cB*(theta + T_ref - Tsub)
+//   + cR*((T_ref + theta)^4 - Tamb^4).
+inline void loss_poly(const StepConst& k, double T_ref, double c[5]) {
+    const double dt = k.dt;
+    const double cH = dt * k.hConv * k.inv_h / k.rho_cp;
+    const double cB = dt * (k.K / k.dSub) * k.inv_h / k.rho_cp;
+    const double cR = k.radiation ? dt * k.eps * 5.670374419e-8 * k.inv_h / k.rho_cp : 0.0;
+    const double Tr = T_ref, Ta = k.Tamb;
+    c[0] = cH * (Tr - Ta) + cB * (Tr - k.Tsub) + cR * (Tr * Tr * Tr * Tr - Ta * Ta * Ta * Ta);
+    c[1] = cH + cB + 4.0 * cR * Tr * Tr * Tr;
+    c[2] = 6.0 * cR * Tr * Tr;
+    c[3] = 4.0 * cR * Tr;
+    c[4] = cR;
+}
+
+// Replaces the diffusion numbers and the centre polynomial of `a`.
+inline void set_coeffs(F32Args& a, double cx, double cy, const double p[5]) {
+    const double kc = 1.0 - 2.0 * (cx + cy);
+    a.cx = static_cast<float>(cx);
+    a.cy = static_cast<float>(cy);
+    a.kc = static_cast<float>(kc);
+    a.kc_d = kc;
+    a.p0 = static_cast<float>(p[0]);
+    a.p1 = static_cast<float>(p[1]);
+    a.p2 = static_cast<float>(p[2]);
+    a.p3 = static_cast<float>(p[3]);
+    a.p4 = static_cast<float>(p[4]);
+    a.a0 = static_cast<float>(-p[0]);
+    a.a1 = static_cast<float>(kc - p[1]);
+    a.a2 = static_cast<float>(-p[2]);
+    a.a3 = static_cast<float>(-p[3]);
+    a.a4 = static_cast<float>(-p[4]);
+}
+
 inline F32Args make_args(const float* in, float* out, int ld, const StepConst& k, double T_ref,
                          int bc, double T_bc, double Tm, int depth, int rows, void* work) {
     F32Args a{};
@@ -588,28 +624,21 @@ inline F32Args make_args(const float* in, float* out, int ld, const StepConst& k
     const double dt = k.dt;
     a.cx = static_cast<float>(k.alpha * dt * k.inv_dx2);
     a.cy = static_cast<float>(k.alpha * dt * k.inv_dy2);
-    a.kc = static_cast<float>(1.0 - 2.0 * k.alpha * dt * (k.inv_dx2 + k.inv_dy2));
-    // dt*loss in K as a polynomial in theta = T - T_ref:
-    //   cH*(theta + T_ref - Tamb) + cB*(theta + T_ref - Tsub)
-    //   + cR*((T_ref + theta)^4 - Tamb^4)
-    const double cH = dt * k.hConv * k.inv_h / k.rho_cp;
-    const double cB = dt * (k.K / k.dSub) * k.inv_h / k.rho_cp;
-    const double cR = k.radiation ? dt * k.eps * 5.670374419e-8 * k.inv_h / k.rho_cp : 0.0;
-    const double Tr = T_ref, Ta = k.Tamb;
-    a.a0 = static_cast<float>(cH * (Tr - Ta) + cB * (Tr - k.Tsub) +
-                              cR * (Tr * Tr * Tr * Tr - Ta * Ta * Ta * Ta));
-    a.a1 = static_cast<float>(cH + cB + 4.0 * cR * Tr * Tr * Tr);
-    a.a2 = static_cast<float>(6.0 * cR * Tr * Tr);
-    a.a3 = static_cast<float>(4.0 * cR * Tr);
-    a.a4 = static_cast<float>(cR);
     const double kc = 1.0 - 2.0 * k.alpha * dt * (k.inv_dx2 + k.inv_dy2);
+    a.kc = static_cast<float>(kc);
     a.kc_d = kc;
-    a.p0 = static_cast<float>(-(cH * (Tr - Ta) + cB * (Tr - k.Tsub) +
-                                cR * (Tr * Tr * Tr * Tr - Ta * Ta * Ta * Ta)));
-    a.p1 = static_cast<float>(kc - (cH + cB + 4.0 * cR * Tr * Tr * Tr));
-    a.p2 = static_cast<float>(-6.0 * cR * Tr * Tr);
-    a.p3 = static_cast<float>(-4.0 * cR * Tr);
-    a.p4 = static_cast<float>(-cR);
+    double L[5];
+    loss_poly(k, T_ref, L);
+    a.a0 = static_cast<float>(L[0]);
+    a.a1 = static_cast<float>(L[1]);
+    a.a2 = static_cast<float>(L[2]);
+    a.a3 = static_cast<float>(L[3]);
+    a.a4 = static_cast<float>(L[4]);
+    a.p0 = static_cast<float>(-L[0]);
+    a.p1 = static_cast<float>(kc - L[1]);
+    a.p2 = static_cast<float>(-L[2]);
+    a.p3 = static_cast<float>(-L[3]);
+    a.p4 = static_cast<float>(-L[4]);
     a.theta_m = static_cast<float>(Tm - T_ref);
     a.theta_bc = static_cast<float>(T_bc - T_ref);
     const double log2e = 1.4426950408889634;

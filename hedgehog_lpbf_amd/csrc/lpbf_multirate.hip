@@ -66,6 +66,31 @@ __device__ inline float prolong_block(const float* __restrict__ B, int ldb, int 
     return 0.5625f * a + 0.1875f * (b + c) + 0.0625f * d;
 }
 
+// Conservative quadratic prolongation at fine cell (X, Y), for the field the
+// run carries (FD margin, cells the window gains, the field it reports): per
+// axis, the quadratic through three neighbouring coarse cell means, averaged
+// over the fine half cell, is c + (c_prev - c_next) / 8 on the lower half and
+// c + (c_next - c_prev) / 8 on the upper; the 2D value is the tensor product.
+// The two halves average to c (a restriction returns the coarse field), and
+// a quadratic field is reproduced in cell means.  Indices clamp at the plate
+// edge (the mirror ghost).  Behind the laser the trail is a ridge a few
+// coarse cells wide, where the bilinear form's error is about ten times
+// larger.
+__device__ inline float prolong_q(const float* __restrict__ C, int ldc, int nxc, int nyc, int X,
+                                  int Y) {
+    const int cx = X >> 1, cy = Y >> 1;
+    const int xm = max(cx - 1, 0), xp = min(cx + 1, nxc - 1);
+    const int ym = max(cy - 1, 0), yp = min(cy + 1, nyc - 1);
+    const float sx = (X & 1) ? 0.125f : -0.125f;
+    const float sy = (Y & 1) ? 0.125f : -0.125f;
+    auto row = [&](int v) {
+        const float* r = C + static_cast<size_t>(v) * ldc;
+        return fmaf(sx, r[xp] - r[xm], r[cx]);
+    };
+    const float lo = row(ym), mid = row(cy), hi = row(yp);
+    return fmaf(sy, hi - lo, mid);
+}
+
 // The 2x2 mean, in one fixed order everywhere (plate, window, residual input).
 __device__ inline float mean4(float a, float b, float c, float d) {
     return 0.25f * ((a + b) + (c + d));
@@ -158,8 +183,8 @@ __global__ __launch_bounds__(kT) void k_mr_target(const lpbf_mr_window* __restri
 // the region 16 cells wider on every side (clipped at the plate edge, where
 // the clamped neighbour index is the mirror ghost): after 16 steps the
 // region's edge error has reached 15 cells in, so the owned cells are exact.
-// The region starts from the window's cells and, outside it, the prolonged
-// coarse field.  Every cell takes the FP32 step kernel's expression
+// The region starts from the window's cells and, outside it, the coarse
+// field's quadratic prolongation.  Every cell takes the FP32 step kernel's expression
 // (theta' = P(theta) + source + cy*(up+down) + cx*(left+right), P the
 // centre part kc*theta - loss as one polynomial).
 constexpr int kFdTile = 16;
@@ -191,7 +216,7 @@ __global__ __launch_bounds__(kT) void k_mr_fd_window(F32Args a, const float* __r
             const int wi = Y - oy, wj = X - ox;
             buf[0][r * kFdReg + c] = (wi >= 0 && wi < kW && wj >= 0 && wj < kW)
                                          ? Wc[wi * kW + wj]
-                                         : prolong(C, ldc, nxc, nyc, X, Y);
+                                         : prolong_q(C, ldc, nxc, nyc, X, Y);
         }
     __syncthreads();
     const int cl = max(c - 1, 0), cr = min(c + 1, RW - 1);
@@ -241,9 +266,10 @@ __global__ __launch_bounds__(kT) void k_mr_fd_window(F32Args a, const float* __r
 // prolongation reads (kCwOut^2 from (bx, by) = window origin / 2 - 1), for
 // the network's input without waiting for the plate's coarse step: the
 // region 4 cells wider on every side (clipped at the plate edge) is stepped
-// in LDS with the coarse step kernel's expression from the plate at t.
-// Cells outside the plate are not written (the prolongation never reads
-// them).
+// in LDS with the coarse step kernel's expression from the plate at t, with
+// the plain coarse coefficients (dt x 4 in one explicit step), which define
+// the increment the network was trained to correct.  Cells outside the
+// plate are not written (the prolongation never reads them).
 constexpr int kCwOut = kWc + 2;                 // 66
 constexpr int kCwReg = kCwOut + 2 * 4;          // 74
 constexpr int kCwThreads = 1024;
@@ -337,8 +363,8 @@ __global__ __launch_bounds__(kT) void k_mr_compose(const float* __restrict__ Wc,
 // After a block: the window's new cells (src) overwrite their coarse cells
 // (2x2 means, all but m coarse cells at each window edge), and the window
 // moves to its next origin: cells it keeps come from src, the others are
-// prolonged from the coarse grid.  The two parts are independent: a cell
-// outside the old window is prolonged from coarse cells at most one cell
+// prolonged (quadratic) from the coarse grid.  The two parts are independent:
+// a cell outside the old window is prolonged from coarse cells at most one cell
 // inside the old window's coarse region, which lie in the unwritten edge
 // band (m >= 1).  src is either a materialised window (Ws) or, with Ws
 // null, the network result Wc + D + ramp * 100 y, formed on the fly.
@@ -366,7 +392,7 @@ __global__ __launch_bounds__(kT) void k_mr_post(PostSrc src, int ox0, int oy0, i
         const int X = ox1 + j, Y = oy1 + i;
         const int wi = Y - oy0, wj = X - ox0;
         Wn[q] = (wi >= 0 && wi < kW && wj >= 0 && wj < kW) ? src(wi, wj)
-                                                          : prolong(C, ldc, nxc, nyc, X, Y);
+                                                          : prolong_q(C, ldc, nxc, nyc, X, Y);
         return;
     }
     const int n = kWc - 2 * m;
@@ -389,9 +415,49 @@ __global__ __launch_bounds__(kT) void k_mr_field(const float* __restrict__ Wc, i
         const int wi = Y - oy, wj = X - ox;
         const float v = (wi >= 0 && wi < kW && wj >= 0 && wj < kW)
                             ? Wc[wi * kW + wj]
-                            : prolong(C, ldc, nxc, nyc, X, Y);
+                            : prolong_q(C, ldc, nxc, nyc, X, Y);
         T[q] = T_ref + static_cast<double>(v);
     }
+}
+
+// g(f(theta)) for quartics g, f, terms above theta^4 dropped.
+void compose4(const double g[5], const double f[5], double out[5]) {
+    double pw[5] = {1.0, 0.0, 0.0, 0.0, 0.0};   // f^d
+    for (int i = 0; i < 5; ++i) out[i] = 0.0;
+    for (int d = 0; d <= 4; ++d) {
+        for (int i = 0; i < 5; ++i) out[i] += g[d] * pw[i];
+        double nx[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < 5; ++i)
+            for (int j = 0; i + j < 5; ++j) nx[i + j] += pw[i] * f[j];
+        for (int i = 0; i < 5; ++i) pw[i] = nx[i];
+    }
+}
+
+// The coarse step's coefficients, chosen so that one coarse step (dt x 4,
+// dx x 2) reproduces four fine explicit steps rather than one explicit step
+// of four times the length.  With the fine step theta' = Phi(theta) + D theta
+// (Phi(theta) = theta - dt loss(theta) pointwise, D the diffusion), four fine
+// steps are Phi^4(theta) + 4 Phi'^3 D theta + O(D^2): the coarse centre part
+// is Phi^4 (composed in FP64; the dropped terms above theta^4 come to
+// 5e-6 K at 3000 K) and its diffusion numbers carry the factor
+// (1 - dt loss'(0))^3.  A plain coarse step's loss term, 1 - 4 lambda dt
+// against (1 - lambda dt)^4, drifts by 6 (lambda dt)^2 per coarse step: about
+// 4 K over the millisecond a hot track takes to cool on a 50 um plate.
+F32Coeffs coarse_coeffs(const StepConst& kf, double T_ref) {
+    double L[5];
+    loss_poly(kf, T_ref, L);
+    const double phi[5] = {-L[0], 1.0 - L[1], -L[2], -L[3], -L[4]};
+    double phi2[5], phi4[5];
+    compose4(phi, phi, phi2);
+    compose4(phi2, phi2, phi4);
+    const double g = (1.0 - L[1]) * (1.0 - L[1]) * (1.0 - L[1]);
+    F32Coeffs c;
+    // the coarse grid's diffusion numbers equal the fine grid's (dt x 4, dx^2 x 4)
+    c.cx = kf.alpha * kf.dt * kf.inv_dx2 * g;
+    c.cy = kf.alpha * kf.dt * kf.inv_dy2 * g;
+    for (int i = 0; i < 5; ++i) c.p[i] = phi4[i];
+    c.p[1] -= 2.0 * (c.cx + c.cy);
+    return c;
 }
 
 int grid_for(long long n) {
@@ -422,7 +488,8 @@ struct lpbf_mr_run {
     int nx = 0, ny = 0, nxc = 0, nyc = 0, ldc = 0, n_blocks = 0, depth_c = 4;
     std::vector<int> ox, oy;             // window origin per block (n_blocks + 1)
     F32Args af{};                        // the fine step's constants
-    F32Args ac{};                        // the coarse step's constants
+    F32Args ac{};                        // the plain coarse step's constants (network input)
+    F32Coeffs ccoef{};                   // the plate's coarse step (four fine steps)
     hipStream_t s = nullptr;             // the caller's stream: the window
     hipStream_t s2 = nullptr;            // the coarse plate's steps
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
@@ -492,9 +559,8 @@ struct lpbf_mr_run {
         for (size_t l = 0; l < ds.size(); ++l) {
             float* out = (l + 1 == ds.size() || l % 2) ? C[cc ^ 1] : C2;
             // the statistics switch is off: Tm only feeds the molten count
-            lpbf_launch_step_f32(in, out, ldc, &kc, cfg.T_ref, LPBF_BC_MIRROR, cfg.T_ref,
-                                 cfg.T_ref + 1e9, d_cplan, 4 * b + t, ds[l], cfg.coarse_rows,
-                                 nullptr, work_c, s2);
+            step_f32_launch_coeffs(in, out, ldc, kc, cfg.T_ref, ccoef, d_cplan, 4 * b + t, ds[l],
+                                   cfg.coarse_rows, work_c, s2);
             in = out;
             t += ds[l];
         }
@@ -650,6 +716,16 @@ int lpbf_mr_window_origins(const LaserTick* h_plan, int n_blocks, int nx, int ny
     return 0;
 }
 
+int lpbf_mr_coarse_coeffs(const StepConst* k, double T_ref, double* out7) {
+    if (!k || !out7 || !(k->dt > 0.0) || !(k->dx > 0.0) || !(k->dy > 0.0))
+        return fail(LPBF_ERR_INVALID_ARGUMENT);
+    const F32Coeffs c = coarse_coeffs(*k, T_ref);
+    out7[0] = c.cx;
+    out7[1] = c.cy;
+    for (int i = 0; i < 5; ++i) out7[2 + i] = c.p[i];
+    return 0;
+}
+
 int lpbf_mr_create(const lpbf_mr_config* cfg, const LaserTick* h_plan, const void* d_pack,
                    hipStream_t stream, lpbf_mr_run** out) {
     if (!cfg || !h_plan || !out) return fail(LPBF_ERR_INVALID_ARGUMENT);
@@ -695,8 +771,12 @@ int lpbf_mr_create(const lpbf_mr_config* cfg, const LaserTick* h_plan, const voi
             throw Failure{LPBF_ERR_INVALID_ARGUMENT};
         r->af = make_args(nullptr, nullptr, k.nx, k, cfg->T_ref, LPBF_BC_MIRROR, cfg->T_ref,
                           cfg->T_ref + 1e9, 1, 0, nullptr);
+        // the window-local coarse step (the network's coarse increment) keeps
+        // the plain coarse step the network was trained on; the plate's
+        // coarse step takes the four-fine-step coefficients
         r->ac = make_args(nullptr, nullptr, r->ldc, kc, cfg->T_ref, LPBF_BC_MIRROR, cfg->T_ref,
                           cfg->T_ref + 1e9, 1, 0, nullptr);
+        r->ccoef = coarse_coeffs(k, cfg->T_ref);
         hip_ok(hipStreamCreateWithFlags(&r->s2, hipStreamNonBlocking));
         hip_ok(hipEventCreateWithFlags(&r->ev_a, hipEventDisableTiming));
         hip_ok(hipEventCreateWithFlags(&r->ev_b, hipEventDisableTiming));
@@ -750,9 +830,8 @@ int lpbf_mr_create(const lpbf_mr_config* cfg, const LaserTick* h_plan, const voi
                 for (int R : kRows) {
                     if (R > r->nyc) continue;
                     auto go = [&] {
-                        lpbf_launch_step_f32(r->C[0], r->C[1], r->ldc, &r->kc, cfg->T_ref,
-                                             LPBF_BC_MIRROR, cfg->T_ref, cfg->T_ref + 1e9,
-                                             r->d_cplan, 0, 4, R, nullptr, work, stream);
+                        step_f32_launch_coeffs(r->C[0], r->C[1], r->ldc, r->kc, cfg->T_ref,
+                                               r->ccoef, r->d_cplan, 0, 4, R, work, stream);
                     };
                     go();
                     hip_ok(hipEventRecord(e0, stream));

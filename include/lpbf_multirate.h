@@ -18,15 +18,21 @@
  *             residual (fine increment - prolonged coarse increment) / 100 K
  *             from four input channels (below); its weight rises from 0 at
  *             the window edge to 1 at `overlap` cells in;
- *   FD window the fine step itself on the window plus 16 cells of prolonged
- *             coarse field on every side (the 16 steps cannot carry the
- *             margin's edge error into the window), stepped in LDS by one
+ *   FD window the fine step itself on the window plus 16 cells of coarse
+ *             field on every side, prolonged conservatively (per axis the
+ *             quadratic through three coarse cell means; the 16 steps cannot
+ *             carry the margin's edge error into the window), stepped in LDS by one
  *             launch: the network's judge every `recheck_every` calls and
  *             its fallback;
+ *   coarse    the plate's coarse step takes lpbf_mr_coarse_coeffs (four fine
+ *             steps' loss and diffusion in one step); the window-local coarse
+ *             step that feeds the network keeps the plain coarse step (dt x 4
+ *             in one explicit step), the increment it was trained on;
  *   coupling  after each window advance its cells overwrite their coarse
  *             cells (2x2 mean, all but `overlap`/2 coarse cells at the
  *             window edge), and when the window moves the cells it gains are
- *             prolonged from the coarse grid.
+ *             prolonged (quadratic) from the coarse grid.  The network's
+ *             channels and base use the bilinear form it was trained with.
  *
  * Network input channels (bf16, the 4 of the 32-channel layer-1 input):
  *   0  (theta - P R theta) / 100 K: the fine-scale part of the window field
@@ -98,6 +104,16 @@ void lpbf_mr_target(const lpbf_mr_window* d_win, int n, const float* d_T0, const
 int  lpbf_mr_window_origins(const LaserTick* h_plan, int n_blocks, int nx, int ny, double dx,
                             double dy, int* ox, int* oy);
 
+/* The coarse plate's step coefficients for the fine constants k (host
+ * arithmetic, no device work).  One coarse step (dt x 4, dx x 2) reproduces
+ * four fine explicit steps: with the fine step theta' = Phi(theta) + D theta
+ * (Phi(theta) = theta - dt loss(theta), D the diffusion), the centre part is
+ * Phi composed four times (quartic, FP64) less 2 (cx + cy) theta, and the
+ * diffusion numbers are the fine ones times (1 - dt loss'(0))^3.
+ * out[0] = cx, out[1] = cy, out[2 + i] = coefficient of theta^i of the centre
+ * polynomial (theta' = P(theta) + cy (up + down) + cx (left + right) + source). */
+int  lpbf_mr_coarse_coeffs(const StepConst* k, double T_ref, double* out7);
+
 /* ---- the two-level run ----------------------------------------------------- */
 
 typedef struct lpbf_mr_config {
@@ -137,7 +153,8 @@ void lpbf_mr_destroy(lpbf_mr_run* r);
  * synchronises `stream` at each recheck to take the fallback decision. */
 int  lpbf_mr_advance(lpbf_mr_run* r, int n_blocks);
 /* The run's field on the fine grid (FP64 T, nx*ny row-major, device): the
- * window's cells where it lies, the prolonged coarse field elsewhere. */
+ * window's cells where it lies, the coarse field's quadratic prolongation
+ * elsewhere. */
 int  lpbf_mr_field(const lpbf_mr_run* r, double* d_T);
 int  lpbf_mr_get_counters(const lpbf_mr_run* r, lpbf_mr_counters* out);
 /* Up to max recheck differences in kelvin, in order; returns their count. */
