@@ -119,11 +119,6 @@ def counters_for(workload, **launch):
     return None
 
 
-def traffic_for(workload, **launch):
-    t = counters_for(workload, **launch)
-    return t.get("bytes_per_launch") if t else None
-
-
 def batch_setup(n_inst=256, n_steps=5000, seed=0):
     """256 instances, sides log-uniform in [256, 2048] per axis, P ~ U[100, 400] W,
     V ~ U[0.5, 2] m/s, r0 ~ U[35, 70] um, dx = 10 um, dt = 0.8 * dx^2 / (4 alpha),
@@ -172,10 +167,18 @@ def batch_bench(args):
     b = h.F32Batch(specs, bc=0)
     s = torch.cuda.Stream()
     t0 = time.perf_counter()
-    ms_tuned = b.autotune(4, s.cuda_stream)
-    tiling = {c: b.tiling(c) for c in (0, 1, 2)}
-    print(f"[bench] autotune {time.perf_counter() - t0:.1f} s: {tiling} "
-          f"({ms_tuned:.3f} ms/step summed over classes)", file=sys.stderr, flush=True)
+    if args.batch_tiling:
+        # fixed tiling per size class, "depth:rows" for classes 0, 1, 2
+        for c, dr in enumerate(args.batch_tiling.split(",")):
+            d, r = (int(v) for v in dr.split(":"))
+            b.set_tiling(c, d, r)
+        tiling = {c: b.tiling(c) for c in (0, 1, 2)}
+        print(f"[bench] fixed tiling {tiling}", file=sys.stderr, flush=True)
+    else:
+        ms_tuned = b.autotune(4, s.cuda_stream)
+        tiling = {c: b.tiling(c) for c in (0, 1, 2)}
+        print(f"[bench] autotune {time.perf_counter() - t0:.1f} s: {tiling} "
+              f"({ms_tuned:.3f} ms/step for the schedule)", file=sys.stderr, flush=True)
     with torch.cuda.stream(s):
         b.enqueue(0, warmup, s.cuda_stream)
         b.capture(0, steps, s.cuda_stream)
@@ -208,19 +211,36 @@ def batch_bench(args):
                 kk = ora.StepConst.from_buffer_copy(bytes(sp.k))
                 T = np.full((sp.k.ny, sp.k.nx), 300.0, dtype=np.float32)
                 out = np.empty_like(T)
-                for q in range(10):
+                for q in range(40):
                     kk.x_l, kk.y_l, kk.q_scale = sp.plan[q]
                     ora.step_threads_f32(T, kk, threads=nt, out=out, pool=pool)
                     T, out = out, T
-                done += 10 * sp.k.nx * sp.k.ny
+                done += 40 * sp.k.nx * sp.k.ny
                 n_inst += 1
                 if time.perf_counter() - t0 > 12.0:
                     break
             el = time.perf_counter() - t0
         cpu = {"value": done / el, "unit": "cell-updates/s", "cores": nt, "kind": "port",
                "sample": f"oracle FP32 step (C, -O3) on {nt} host threads, first {n_inst} "
-                         f"instances x 10 steps, {el:.1f} s"}
-    achieved = 8.0 * cells * steps / (total_ms * 1e-3) / 1e9
+                         f"instances x 40 steps, {el:.1f} s"}
+    # roofline: the step kernels are issue-bound like the single plate's.
+    # HBM bytes and VALU instructions per cell-update come from the committed
+    # counter summary (every step launch of one graph replay, summed; they
+    # vary by a few percent with the band heights the autotuner picks), else
+    # the compulsory bytes: one read and one write of every instance per
+    # launch.
+    prof = counters_for("batch256")
+    rate = cells * steps / (total_ms * 1e-3)            # cell-updates/s
+    depth_of = {c: d for c, (d, r, m) in tiling.items()}
+    compulsory = sum(8.0 * sp.k.nx * sp.k.ny / depth_of[sp.size_class] for sp in specs)
+    if prof:
+        bytes_step = prof["hbm_bytes_per_cell_update"] * cells
+        basis = (f"counters (FETCH_SIZE + WRITE_SIZE, copy-calibrated) per cell-update, "
+                 f"measured at tiling {prof['tiling']}")
+        valu_frac = prof["valu_wave_insts_per_cell_update"] * rate / VALU_ISSUE_PER_S
+    else:
+        bytes_step, basis, valu_frac = compulsory, "compulsory bytes (8 B x cells / depth)", None
+    achieved = bytes_step * steps / (total_ms * 1e-3) / 1e9
     out = {
         "metric": "cell-updates/s", "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": steps, "warmup": warmup, "ms_per_step": total_ms / steps,
@@ -235,9 +255,14 @@ def batch_bench(args):
                                  for c, (d, r, m) in tiling.items()},
         },
         "roofline": {
-            "bound": "hbm", "kernel": "k_step_f32_multi (whole graph incl. stats merges)",
+            "bound": prof.get("bound", "valu") if prof else "valu",
+            "kernel": "k_step_f32_multi (whole graph incl. stats merges)",
             "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for("batch256"),
+            "frac": achieved / HBM_PEAK_GBS, "traffic": bytes_step,
+            "traffic_basis": basis, "traffic_unit": "bytes per step of all instances",
+            "valu_frac": valu_frac,
+            "alg_frac": 8.0 * rate / (HBM_PEAK_GBS * 1e9),
+            "counter_summary": prof.get("summary") if prof else None,
         },
         "cpu_baseline": cpu,
     }
@@ -549,7 +574,7 @@ def plate_measure(args, precision, min_window_ms=200.0):
     window), repeated until `min_window_ms` of timed GPU work so that a short
     driver run is not a single millisecond; value from the median window.
     The step kernel's average launch duration comes from HIP events on its
-    stream around the full-depth launches."""
+    stream around every tenth full-depth launch of the timed windows."""
     import numpy as np
     import torch
 
@@ -624,25 +649,31 @@ def plate_measure(args, precision, min_window_ms=200.0):
         kernel_name = "k_step_f64_tb"
     bufs = [a, b]
 
-    def run(s0, n, events=None):
-        s = s0
+    def run(s0, n, events=None, every=10):
+        # events: pre-created (start, end, depth) slots recorded on the launch
+        # stream around every `every`-th step launch (sparse, so that the
+        # markers barely perturb the window they sit in)
+        s, q, i = s0, 0, 0
         while s < s0 + n:
             d = min(depth, s0 + n - s)
-            if events is not None:
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
+            timed = events is not None and i % every == 0 and q < len(events)
+            if timed:
+                events[q][0].record(stream)
             launch(s, d, bufs[0], bufs[1])
-            if events is not None:
-                e1.record(stream)
-                events.append((e0, e1, d))
+            if timed:
+                events[q][1].record(stream)
+                events[q][2] = d
+                q += 1
             finalize(s, d)
             bufs.reverse()
             s += d
+            i += 1
 
     run(0, args.warmup)
     torch.cuda.synchronize(dev)
     h.check()
+    # timed windows: events around every tenth launch give the step kernel's
+    # duration in steady state (created before the window starts)
     windows, events = [], []
     s0 = args.warmup
     while True:
@@ -650,9 +681,12 @@ def plate_measure(args, precision, min_window_ms=200.0):
             break
         t_start = torch.cuda.Event(enable_timing=True)
         t_end = torch.cuda.Event(enable_timing=True)
+        ev = [[torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True), 0]
+              for _ in range(-(-args.steps // depth) // 10 + 1)]
+        events += ev
         torch.cuda.synchronize(dev)
         t_start.record(stream)
-        run(s0, args.steps, events)
+        run(s0, args.steps, ev)
         t_end.record(stream)
         torch.cuda.synchronize(dev)
         windows.append(t_start.elapsed_time(t_end))
@@ -663,10 +697,10 @@ def plate_measure(args, precision, min_window_ms=200.0):
     if not windows:
         raise SystemExit(f"--warmup + --steps exceed the {plan.shape[0]}-step scan plan")
     win_ms = float(np.median(windows))
-    full = [e0.elapsed_time(e1) for e0, e1, d in events if d == depth]
-    kern_ms = float(np.mean(full)) if full else float("nan")
     last = (h.StepStats * 1).from_buffer_copy(
         stats.cpu().numpy().tobytes()[56 * (s0 - 1): 56 * s0])[0]
+    full = [e0.elapsed_time(e1) for e0, e1, d in events if d == depth]
+    kern_ms = float(np.mean(full)) if full else float("nan")
     print(f"[bench] {len(windows)} windows of {args.steps} steps, median {win_ms:.3f} ms; "
           f"step-kernel avg {kern_ms:.4f} ms per launch of {depth} steps; peak T "
           f"{last.tmax:.1f} K, molten {last.molten}", file=sys.stderr, flush=True)
@@ -734,6 +768,9 @@ def main():
     ap.add_argument("--n", type=int, default=8192, help="plate side (cells)")
     ap.add_argument("--runs", type=int, default=64, help="ensemble: manifest rows")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-tiling", default="",
+                    help="batch256: fixed depth:rows for size classes 0,1,2 (e.g. "
+                         "4:16,3:48,3:48) instead of the autotuner")
     ap.add_argument("--no-fp64", action="store_true",
                     help="plate8k_f32: leave out the FP64 parity-path object")
     ap.add_argument("--data-steps", type=int, default=49168,

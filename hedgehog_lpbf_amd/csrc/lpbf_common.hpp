@@ -85,7 +85,7 @@ struct PartialAcc {
     __host__ __device__ void take_max(double v, long long a) {
         if (v > tmax || (v == tmax && a < arg)) { tmax = v; arg = a; }
     }
-    __host__ __device__ void merge(const PartialAcc& o) {
+    __host__ __device__ __forceinline__ void merge(const PartialAcc& o) {
         take_max(o.tmax, o.arg);
         if (o.ymin < ymin) ymin = o.ymin;
         if (o.ymax > ymax) ymax = o.ymax;

@@ -461,45 +461,144 @@ int lpbf_f32_batch_autotune(lpbf_f32_batch* b, int max_depth, hipStream_t stream
         lpbf_amd::ok(hipEventCreate(&e0));
         lpbf_amd::ok(hipEventCreate(&e1));
         static const int kRows[] = {0, 16, 24, 32, 40, 48, 64, 96, 128};
+        constexpr double kTie = 1.02;   // timings this close count as equal
         const int d_first = max_depth == 1 ? 1 : 2;
+        // ms per step of one launch group (two windows of six launches,
+        // the faster kept)
+        auto time_group = [&](std::vector<int> const& members, int d, bool rad, int rows) {
+            Batch::Group g = B.build(members, d, rad, rows, false);
+            double t = 1e300;
+            try {
+                for (int w = 0; w < 2; ++w) {
+                    B.launch(g, d, 0, stream);   // warm-up
+                    lpbf_amd::ok(hipEventRecord(e0, stream));
+                    for (int q = 0; q < 6; ++q) B.launch(g, d, 0, stream);
+                    lpbf_amd::ok(hipEventRecord(e1, stream));
+                    lpbf_amd::ok(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    lpbf_amd::ok(hipEventElapsedTime(&ms, e0, e1));
+                    t = std::min(t, ms / (6.0 * d));
+                }
+            } catch (...) {
+                Batch::free_group(g);
+                throw;
+            }
+            Batch::free_group(g);
+            return t;
+        };
+        // the launch groups of the current tiling: classes of equal depth
+        // (and radiation switch) share a launch
+        auto groups_now = [&] {
+            std::map<std::pair<int, bool>, std::vector<int>> sh;
+            for (size_t i = 0; i < B.inst.size(); ++i)
+                sh[{B.depth_of(B.inst[i].size_class), B.rad_of(static_cast<int>(i))}].push_back(
+                    static_cast<int>(i));
+            return sh;
+        };
+        auto schedule_ms = [&] {
+            double t = 0.0;
+            for (auto const& sg : groups_now())
+                t += time_group(sg.second, sg.first.first, sg.first.second, -1);
+            return t;
+        };
+
+        // 1. each class alone: per depth, the fastest band height (within
+        //    timing noise of it, the tallest: fewer halo rows and less
+        //    traffic, which tells most when classes share the device)
+        std::map<int, std::map<int, std::pair<double, int>>> at;   // class -> depth -> (ms, rows)
+        std::map<int, int> min_ny;
         for (auto& kv : B.classes) {
             std::map<bool, std::vector<int>> mem;   // by radiation switch
+            int mny = 1 << 30;
             for (size_t i = 0; i < B.inst.size(); ++i)
-                if (B.inst[i].size_class == kv.first)
+                if (B.inst[i].size_class == kv.first) {
                     mem[B.rad_of(static_cast<int>(i))].push_back(static_cast<int>(i));
-            int min_ny = 1 << 30;
-            for (auto const& m : mem)
-                for (int i : m.second) min_ny = std::min(min_ny, B.inst[static_cast<size_t>(i)].k.ny);
-            double best = 1e300;
-            int bd = max_depth, br = 0;
-            for (int d = d_first; d <= max_depth; ++d)
-                for (int R : kRows) {
-                    if (R > 0 && (R < 2 * d || R > min_ny)) continue;
-                    double ms_sum = 0.0;
-                    const int reps = 6;   // enough launches that timing noise does not pick
-                    for (auto const& m : mem) {
-                        Batch::Group g = B.build(m.second, d, m.first, R, false);
-                        try {
-                            B.launch(g, d, 0, stream);   // warm-up
-                            lpbf_amd::ok(hipEventRecord(e0, stream));
-                            for (int q = 0; q < reps; ++q) B.launch(g, d, 0, stream);
-                            lpbf_amd::ok(hipEventRecord(e1, stream));
-                            lpbf_amd::ok(hipEventSynchronize(e1));
-                        } catch (...) {
-                            Batch::free_group(g);
-                            throw;
-                        }
-                        float ms = 0.f;
-                        lpbf_amd::ok(hipEventElapsedTime(&ms, e0, e1));
-                        ms_sum += ms / (static_cast<double>(reps) * d);
-                        Batch::free_group(g);
-                    }
-                    if (ms_sum < best) { best = ms_sum; bd = d; br = R; }
+                    mny = std::min(mny, B.inst[i].k.ny);
                 }
-            kv.second.depth = bd;
-            kv.second.rows = br;
-            total += best;
+            min_ny[kv.first] = mny;
+            for (int d = d_first; d <= max_depth; ++d) {
+                std::vector<std::pair<double, int>> timed;
+                for (int R : kRows) {
+                    if (R > 0 && (R < 2 * d || R > mny)) continue;
+                    double ms = 0.0;
+                    for (auto const& m : mem) ms += time_group(m.second, d, m.first, R);
+                    timed.push_back({ms, R});
+                }
+                std::pair<double, int> b{1e300, 0};
+                for (auto const& t : timed)
+                    if (t.first < b.first) b = t;
+                for (auto const& t : timed)
+                    if (t.first <= b.first * kTie && t.second > b.second && b.second != 0)
+                        b.second = t.second;
+                at[kv.first][d] = b;
+            }
         }
+        // 2. schedules: every class at its own best depth, or every class at
+        //    one depth (one shared launch per step group instead of several
+        //    small ones); each timed as its launches, the fastest kept
+        //    (within timing noise, the one with fewer launch groups)
+        std::vector<std::map<int, std::pair<int, int>>> cands;   // class -> (depth, rows)
+        {
+            std::map<int, std::pair<int, int>> own;
+            for (auto const& c : at) {
+                int bd = d_first;
+                for (auto const& dv : c.second)
+                    if (dv.second.first < c.second.at(bd).first) bd = dv.first;
+                own[c.first] = {bd, c.second.at(bd).second};
+            }
+            cands.push_back(own);
+            for (int d = d_first; d <= max_depth; ++d) {
+                std::map<int, std::pair<int, int>> one;
+                for (auto const& c : at) one[c.first] = {d, c.second.at(d).second};
+                if (one != own) cands.push_back(one);
+            }
+        }
+        double best_ms = 1e300;
+        size_t best_groups = 0;
+        std::map<int, std::pair<int, int>> pick;
+        for (auto const& cand : cands) {
+            for (auto const& c : cand) {
+                B.classes[c.first].depth = c.second.first;
+                B.classes[c.first].rows = c.second.second;
+            }
+            const double t = schedule_ms();
+            const size_t ng = groups_now().size();
+            const bool better = best_ms == 1e300 || t < best_ms / kTie ||
+                                (t <= best_ms * kTie && ng < best_groups);
+            if (better) { best_ms = t; best_groups = ng; pick = cand; }
+        }
+        for (auto const& c : pick) {
+            B.classes[c.first].depth = c.second.first;
+            B.classes[c.first].rows = c.second.second;
+        }
+        // 3. in each shared launch, the band heights picked alone need not be
+        //    best (wave rounds, the halo traffic of the whole launch): one
+        //    pass of coordinate descent, a class's band height at a time,
+        //    timing the shared launch itself
+        for (auto const& sg : groups_now()) {
+            const int d = sg.first.first;
+            std::map<int, int> cls;   // classes in this launch -> smallest ny
+            for (int i : sg.second) cls[B.inst[static_cast<size_t>(i)].size_class] = 0;
+            if (cls.size() < 2) continue;
+            double best = time_group(sg.second, d, sg.first.second, -1);
+            for (auto const& cm : cls) {
+                Batch::Class& cl = B.classes[cm.first];
+                const int keep = cl.rows;
+                std::vector<std::pair<double, int>> timed{{best, keep}};
+                for (int R : kRows) {
+                    if (R == keep || (R > 0 && (R < 2 * d || R > min_ny[cm.first]))) continue;
+                    cl.rows = R;
+                    timed.push_back({time_group(sg.second, d, sg.first.second, -1), R});
+                }
+                int br = keep;
+                for (auto const& t : timed)
+                    if (t.first < best) { best = t.first; br = t.second; }
+                for (auto const& t : timed)
+                    if (t.first <= best * kTie && t.second > br && br != 0) br = t.second;
+                cl.rows = br;
+            }
+        }
+        total = schedule_ms();
         const int e = lpbf_get_last_error();
         if (e) throw Failure{e};
     });

@@ -30,6 +30,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "lpbf_common.hpp"
@@ -109,6 +110,24 @@ __device__ __forceinline__ void set_elem(float4& v, int e, float t) {
     v.y = e == 1 ? t : v.y;
     v.z = e == 2 ? t : v.z;
     v.w = e == 3 ? t : v.w;
+}
+
+// Row loads and stores through device (global address space) pointers: the
+// multi-instance kernel reads its buffer pointers from memory, where the
+// compiler cannot infer the address space and would emit flat accesses,
+// which also count on the scalar-load counter, so that every wait for an
+// instance argument would wait for the row prefetches as well.
+// (a native vector type: float4's copy constructor would bind a generic
+// reference and undo the address space)
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+using gf32x4 = __attribute__((address_space(1))) f32x4;
+__device__ __forceinline__ float4 ld_row4(const float* p) {
+    const f32x4 t = *(const gf32x4*)p;
+    return make_float4(t.x, t.y, t.z, t.w);
+}
+__device__ __forceinline__ void st_row4(float* p, float4 v) {
+    const f32x4 t = {v.x, v.y, v.z, v.w};
+    *(gf32x4*)p = t;
 }
 
 struct LevelAcc {
@@ -250,7 +269,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         auto load_row = [&](int i) {
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
             if (i < y_hi && ld_ok)
-                v = *reinterpret_cast<const float4*>((BUFS ? src : a.in) + static_cast<long long>(i) * a.ld + col4);
+                v = ld_row4((BUFS ? src : a.in) + static_cast<long long>(i) * a.ld + col4);
             return v;
         };
         // rows past y_hi are never consumed, so the row index is clamped
@@ -260,7 +279,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         const int col4c = min(max(col4, 0), a.ld - 4);
         auto load_row_clamped = [&](int i) {
             const int ii = min(i, y_hi - 1);
-            return *reinterpret_cast<const float4*>((BUFS ? src : a.in) + static_cast<long long>(ii) * a.ld + col4c);
+            return ld_row4((BUFS ? src : a.in) + static_cast<long long>(ii) * a.ld + col4c);
         };
 
         // Register rings, indexed by a compile-time phase so the window
@@ -373,7 +392,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             }
             if (own_row) {
                 if (j == K && own_lane)
-                    *reinterpret_cast<float4*>((BUFS ? dst : a.out) + static_cast<long long>(r) * a.ld + col4) = res;
+                    st_row4((BUFS ? dst : a.out) + static_cast<long long>(r) * a.ld + col4, res);
                 LevelAcc& A = acc[j - 1];
                 float4 mx = res;
                 if (ragged && lane == lastl) {
@@ -562,11 +581,17 @@ struct Geometry {
     int nstrips, R, ntasks, nwg;
 };
 
+// The band height is evened out over the bands it implies: `rows` (or the
+// built-in height) sets the band count, ceil(ny / rows), and the bands then
+// take ceil(ny / count) rows each, so the last band is not left short while
+// every other wavefront carries the full height.
 inline Geometry geometry(int nx, int ny, int depth, int rows) {
     Geometry g;
     g.nstrips = (nx + kOutCols - 1) / kOutCols;
-    g.R = rows > 0 ? rows : rows_per_band(nx, ny, depth);
-    g.ntasks = g.nstrips * ((ny + g.R - 1) / g.R);
+    const int R0 = rows > 0 ? std::min(rows, ny) : rows_per_band(nx, ny, depth);
+    const int nbands = (ny + R0 - 1) / R0;
+    g.R = (ny + nbands - 1) / nbands;
+    g.ntasks = g.nstrips * nbands;
     g.nwg = (g.ntasks + kWaves - 1) / kWaves;
     return g;
 }

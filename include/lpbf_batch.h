@@ -63,11 +63,16 @@ int  lpbf_f32_batch_set_tiling(lpbf_f32_batch* b, int size_class, int depth, int
 int  lpbf_f32_batch_get_tiling(const lpbf_f32_batch* b, int size_class, int* depth,
                                int* rows, int* n_members);
 
-/* Times every class alone for depths 2..max_depth (1 if max_depth == 1)
- * and a set of band heights with the instances' own plans from step 0 and
- * keeps the fastest per class.  The fields are saved and restored
- * (synchronises `stream`); *ms_per_step (may be NULL) receives the sum over
- * classes of the best time per step. */
+/* Picks each class's depth and band height by timing, with the instances'
+ * own plans from step 0: (1) every class alone, for depths 2..max_depth (1
+ * if max_depth == 1) and a set of band heights; (2) schedules: every class
+ * at its own best depth, or every class at one depth (so that they share
+ * one launch), each timed as its launches; (3) in each shared launch, each
+ * class's band height again, timing the shared launch.  Timings within 2%
+ * count as equal (then the taller bands, or the schedule with fewer
+ * launches, are kept).  The fields are saved and restored (synchronises
+ * `stream`); *ms_per_step (may be NULL) receives the picked schedule's time
+ * per step, its launches timed one by one. */
 int  lpbf_f32_batch_autotune(lpbf_f32_batch* b, int max_depth, hipStream_t stream,
                              double* ms_per_step);
 
