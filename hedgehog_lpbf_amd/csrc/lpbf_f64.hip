@@ -40,6 +40,8 @@ enum { RED_NONE = 0, RED_PARTIAL = 1, RED_STATS = 2, RED_RING = 3, RED_BATCH = 4
 
 // The rows being updated.  For BC_TILE, in/out point at the first owned row
 // and rows -1 and ny (the halos) exist in both buffers.
+using gdouble = __attribute__((address_space(1))) double;
+
 struct Region {
     const double* in;
     double* out;
@@ -171,12 +173,19 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
         const bool eok = (lane == 0 && c0 > 0) || (lane == kWave - 1 && c0 + kBlockCols < nx);
         const bool tile = (g.bc == BC_TILE);
 
-        const double* rc = g.in + static_cast<long long>(ya) * nx;
+        // rows through global-address-space pointers: the ensemble form reads
+        // its buffer pointers from memory, where the compiler cannot infer
+        // the address space and would use flat accesses, which also count on
+        // the scalar-load counter (every wait for a scalar value would drain
+        // the row prefetches)
+        const gdouble* gin = (const gdouble*)g.in;
+        gdouble* gout = (gdouble*)g.out;
+        const gdouble* rc = gin + static_cast<long long>(ya) * nx;
         double c0v = a0 ? rc[x0] : 0.0, c1v = a1 ? rc[x1] : 0.0;
         double ce = eok ? rc[ecol] : 0.0;
         double p0v = c0v, p1v = c1v;
         if (ya > 0 || tile) {
-            const double* rp = g.in + static_cast<long long>(ya - 1) * nx;
+            const gdouble* rp = gin + static_cast<long long>(ya - 1) * nx;
             p0v = a0 ? rp[x0] : 0.0;
             p1v = a1 ? rp[x1] : 0.0;
         }
@@ -204,7 +213,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
             const int r = ya + 1 + d;
             q0[d] = q1[d] = qe[d] = 0.0;
             if (r <= r_last) {
-                const double* rn = g.in + static_cast<long long>(r) * nx;
+                const gdouble* rn = gin + static_cast<long long>(r) * nx;
                 q0[d] = a0 ? rn[x0] : 0.0;
                 q1[d] = a1 ? rn[x1] : 0.0;
                 qe[d] = eok ? rn[ecol] : 0.0;
@@ -222,7 +231,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
             }
             const int r = y + 1 + kAhead;
             if (r <= r_last) {
-                const double* rn = g.in + static_cast<long long>(r) * nx;
+                const gdouble* rn = gin + static_cast<long long>(r) * nx;
                 q0[kAhead - 1] = a0 ? rn[x0] : 0.0;
                 q1[kAhead - 1] = a1 ? rn[x1] : 0.0;
                 qe[kAhead - 1] = eok ? rn[ecol] : 0.0;
@@ -237,7 +246,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
             const int yg = g.y_off + y;
             const double ypos = static_cast<double>(yg) * k.dy;
             const bool far = beam_far(ypos, bm, k);
-            double* orow = g.out + static_cast<long long>(y) * nx;
+            gdouble* orow = gout + static_cast<long long>(y) * nx;
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 const int x = j ? x1 : x0;

@@ -12,7 +12,7 @@ import pytest
 import hedgehog_lpbf_amd as h
 from tests.devutil import dev, dev_bytes, kconst, plan_dev, stats_np, stream, sync_check
 
-SIZES = [(37, 5), (131, 67), (300, 133), (2, 9), (129, 2)]
+SIZES = [(37, 5), (131, 67), (300, 133), (2, 9), (129, 2), (240, 120), (1100, 150)]
 
 
 def _setup(ora, nx, ny, phys, seed):
