@@ -353,13 +353,14 @@ def case0_bench(args):
         "roofline": None, "cpu_baseline": None}), flush=True)
 
 
-def ensemble_bench(args):
+def ensemble_bench(args, precision=64):
     """The reference's manifest sweep shape: `--runs` rows of the case0 plate
     (240x120, baseline physics, t_end 1.5 ms) with P, V, r0, eta drawn from the
-    reference case table's ranges, stepped together by lpbf::Ensemble (one FP64
-    launch per step for every row).  value = cell-updates of all rows / wall;
-    the same rows one after another on DeviceRun (the row loop) are timed
-    beside it."""
+    reference case table's ranges, stepped together by lpbf::Ensemble (FP64:
+    one launch per step for every row; FP32: the batched kernel, up to 4 steps
+    per launch for every row).  value = cell-updates of all rows / wall; the
+    same rows one after another on DeviceRun of the same precision (the row
+    loop) are timed beside it."""
     import numpy as np
     import torch
 
@@ -378,9 +379,11 @@ def ensemble_bench(args):
     cells = dom.nx * dom.ny * sum(len(r["plan"]) for r in runs)
     reps = args.steps if args.steps is not None else 10
 
-    def ens_once():
+    pk = dict(precision=precision, depth=4) if precision == 32 else {}
+
+    def ens_once(**kw):
         with h.Ensemble(runs, Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, batch=64,
-                        keep_records=False) as e:
+                        keep_records=False, **(kw or pk)) as e:
             t0 = time.perf_counter()
             e.advance(10 ** 9)
             wall = time.perf_counter() - t0
@@ -392,7 +395,7 @@ def ensemble_bench(args):
         for r in runs:
             with h.DeviceRun(r["k"], Tm=mat.Tm, Lx=dom.Lx, Ly=dom.Ly, Tamb=r["Tamb"],
                              plan=r["plan"], x_csv=r["x_csv"], wrapped=r["wrapped"], batch=64,
-                             keep_records=False) as d:
+                             keep_records=False, **pk) as d:
                 d.advance(len(r["plan"]))
                 peaks.append(d.summary()[0])
         return time.perf_counter() - t0, peaks
@@ -405,16 +408,23 @@ def ensemble_bench(args):
     loop = [loop_once() for _ in range(max(1, reps // 5))]
     lwall = sorted(w for w, _ in loop)[len(loop) // 2]
     assert ens[0][1] == loop[0][1], "ensemble and row loop disagree on peak T"
+    extra = {}
+    if precision == 32:   # the FP32 rows against the FP64 parity ensemble
+        p64 = ens_once(precision=64)[1]
+        extra["max_peak_T_diff_vs_fp64_K"] = max(abs(a - b) for a, b in zip(ens[0][1], p64))
     n_steps = max(len(r["plan"]) for r in runs)
     print(json.dumps({
         "metric": "cell-updates/s", "value": cells / wall, "unit": "cell-updates/s",
         "n_gpus": 1, "steps": n_steps, "warmup": 0, "ms_per_step": wall * 1e3 / n_steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if precision == 64 else "f32", "data": "synthetic",
         "config": {"workload": f"manifest sweep: {len(runs)} rows of the case0 plate (240x120, "
                                "baseline physics, 1.5 ms, P U[245,325] W, V U[0.8,1.2] m/s, "
                                "r0 U[24.5,41] um, eta U[0.30,0.45]) stepped together by "
-                               "lpbf::Ensemble, FP64 parity kernels, batch 64",
+                               "lpbf::Ensemble, " + ("FP64 parity kernels, batch 64"
+                                                     if precision == 64 else
+                                                     "batched FP32 kernel, depth 4, batch 64"),
+                   **extra,
                    "median_of": reps, "rows": len(runs),
                    "row_loop_cell_updates_per_s": cells / lwall,
                    "row_loop_wall_ms": lwall * 1e3, "ensemble_wall_ms": wall * 1e3,
@@ -751,12 +761,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="plate8k_f32",
                     choices=["plate8k_f32", "plate8k_f64", "batch256", "case0", "ensemble",
-                             "run8k_f64", "run8k_f32", "surrogate", "copy"],
+                             "ensemble_f32", "run8k_f64", "run8k_f32", "surrogate", "copy"],
                     help="plate8k_f32 (default): the FP32 plate, with the FP64 parity path "
                          "on the same plate in its `fp64` object; plate8k_f64: the FP64 "
                          "line alone; batch256: 256 uneven instances in one hipGraph launch; "
                          "case0: the reference's published case on the device-resident path; "
-                         "ensemble: --runs manifest rows of that plate stepped together; "
+                         "ensemble: --runs manifest rows of that plate stepped together "
+                         "(ensemble_f32: on the batched FP32 kernel); "
                          "run8k_f64 / run8k_f32: the device-resident run on the 8192^2 plate; "
                          "copy: a plain device copy of one n x n FP32 plate per step, the "
                          "reference for calibrating HBM byte counters")
@@ -798,8 +809,8 @@ def main():
         return case0_bench(args)
     if args.workload in ("run8k_f64", "run8k_f32"):
         return run8k_bench(args, 64 if args.workload == "run8k_f64" else 32)
-    if args.workload == "ensemble":
-        return ensemble_bench(args)
+    if args.workload in ("ensemble", "ensemble_f32"):
+        return ensemble_bench(args, 32 if args.workload == "ensemble_f32" else 64)
     if args.workload == "surrogate":
         return surrogate_bench(args)
     if args.steps is None:

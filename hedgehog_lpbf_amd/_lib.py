@@ -78,6 +78,14 @@ class F32Instance(C.Structure):
                 ("d_stats", C.c_void_p), ("n_steps", C.c_int), ("pad_", C.c_int)]
 
 
+class F32ColJob(C.Structure):
+    """include/lpbf_batch.h lpbf_f32_col_job (device pointers as integers)."""
+    _fields_ = [("theta", C.c_void_p * 2), ("stats", C.c_void_p), ("cruise", C.c_void_p),
+                ("col", C.c_void_p), ("T_ref", C.c_double), ("ld", C.c_int), ("nx", C.c_int),
+                ("ny", C.c_int), ("n_steps", C.c_int)]
+
+
+assert C.sizeof(F32ColJob) == 64
 assert C.sizeof(StepConst) == 216 and C.sizeof(Partial) == 24
 assert C.sizeof(StepRecord) == 80 and C.sizeof(F32Instance) == 288
 assert C.sizeof(LaserTick) == 24 and C.sizeof(StepStats) == 56
@@ -175,6 +183,8 @@ SIGNATURES = {
     "lpbf_f32_batch_capture": (_i, [_vp, _i, _i, _vp]),
     "lpbf_f32_batch_launch_graph": (_i, [_vp, _vp]),
     "lpbf_f32_batch_current": (_i, [_vp, _i]),
+    "lpbf_f32_batch_enqueue_launch": (_i, [_vp, _i, _i, _vp]),
+    "lpbf_launch_hot_cols_f32": (_i, [_vp, _i, _i, _i, _i, C.c_longlong, _vp]),
     "lpbf_f32_batch_launches": (_i, [_vp, _i, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     # surrogate (include/lpbf_surrogate.h)
     "lpbf_sg_pack_bytes": (_sz, []),

@@ -18,6 +18,7 @@
 #include <stdexcept>
 #include <utility>
 
+#include "lpbf_batch.h"
 #include "lpbf_host_util.hpp"
 #include "lpbf_kernels.h"
 #include "lpbf_run.h"
@@ -30,6 +31,12 @@ void hip_ok(hipError_t e, const char* what) { hip_check(e, "Ensemble", what); }
 void lib_ok(const char* what) { lib_check("Ensemble", what); }
 
 hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+lpbf_f32_batch* FB(void* b) { return static_cast<lpbf_f32_batch*>(b); }
+
+void rc_ok(int rc, const char* what) {
+    if (rc != 0) lib_check("Ensemble", what);
+    if (rc != 0) throw std::runtime_error(std::string("Ensemble: ") + what + " failed");
+}
 
 }  // namespace
 
@@ -45,6 +52,8 @@ Ensemble::Ensemble(Domain dom, int phys, double Tm, std::vector<EnsembleMember> 
     if (opt_.bc != LPBF_BC_MIRROR && opt_.bc != LPBF_BC_DIRICHLET)
         throw std::invalid_argument("Ensemble: unknown boundary condition");
     if (opt_.batch < 1) throw std::invalid_argument("Ensemble: batch must be >= 1");
+    if (opt_.precision != 64 && opt_.precision != 32)
+        throw std::invalid_argument("Ensemble: precision must be 64 or 32");
     for (EnsembleMember const& r : runs_) {
         if (r.k.nx != nx_ || r.k.ny != ny_)
             throw std::invalid_argument("Ensemble: a run's StepConst grid does not match the domain");
@@ -52,13 +61,23 @@ Ensemble::Ensemble(Domain dom, int phys, double Tm, std::vector<EnsembleMember> 
             throw std::invalid_argument("Ensemble: plan too long");
         max_steps_ = std::max(max_steps_, static_cast<int>(r.plan.size()));
     }
-    if (opt_.rows < 0 || opt_.rows > 1024)
-        throw std::invalid_argument("Ensemble: rows must be 0..1024");
-    if (opt_.rows == 0) opt_.rows = lpbf_step_ensemble_rows(nx_, ny_, n);
-    nblk_ = lpbf_step_ensemble_nblocks(nx_, ny_, opt_.rows);
-    if (nblk_ <= 0) throw std::invalid_argument("Ensemble: grid too large");
-    if (static_cast<long long>(nblk_ + 1) * n > (1LL << 31) - 1)
-        throw std::invalid_argument("Ensemble: too many runs for one launch");
+    if (opt_.precision == 32) {
+        if (phys_ != LPBF_PHYS_BASELINE)
+            throw std::invalid_argument("Ensemble: the FP32 path runs baseline physics only");
+        if (opt_.depth < 1 || opt_.depth > 4)
+            throw std::invalid_argument("Ensemble: depth must be 1..4");
+        if (opt_.rows < 0 || (opt_.rows > 0 && opt_.rows < 2 * opt_.depth))
+            throw std::invalid_argument("Ensemble: rows must be 0 or >= 2*depth");
+        opt_.graph = false;
+    } else {
+        if (opt_.rows < 0 || opt_.rows > 1024)
+            throw std::invalid_argument("Ensemble: rows must be 0..1024");
+        if (opt_.rows == 0) opt_.rows = lpbf_step_ensemble_rows(nx_, ny_, n);
+        nblk_ = lpbf_step_ensemble_nblocks(nx_, ny_, opt_.rows);
+        if (nblk_ <= 0) throw std::invalid_argument("Ensemble: grid too large");
+        if (static_cast<long long>(nblk_ + 1) * n > (1LL << 31) - 1)
+            throw std::invalid_argument("Ensemble: too many runs for one launch");
+    }
     const std::size_t N = static_cast<std::size_t>(nx_) * ny_;
     const std::size_t B = static_cast<std::size_t>(opt_.batch);
     const std::size_t nn = static_cast<std::size_t>(n);
@@ -72,12 +91,7 @@ Ensemble::Ensemble(Domain dom, int phys, double Tm, std::vector<EnsembleMember> 
 
     std::lock_guard<std::mutex> lock(setup_mutex());
     try {
-        auto dmalloc = [&](std::size_t bytes, const char* what) -> void* {
-            void* p = nullptr;
-            hip_ok(hipMalloc(&p, bytes ? bytes : 1), what);
-            dev_.push_back(p);
-            return p;
-        };
+        auto dmalloc = [&](std::size_t bytes, const char* what) { return dalloc(bytes, what); };
         hipStream_t s;
         hip_ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
         stream_ = s;
@@ -98,6 +112,10 @@ Ensemble::Ensemble(Domain dom, int phys, double Tm, std::vector<EnsembleMember> 
             done_ev_[q] = ev;
         }
 
+        if (opt_.precision == 32) {
+            setup_f32();
+            return;
+        }
         std::vector<lpbf_ensemble_run> table(nn);
         std::vector<double> init(N);
         for (std::size_t m = 0; m < nn; ++m) {
@@ -145,10 +163,136 @@ Ensemble::~Ensemble() {
     release();
 }
 
+void* Ensemble::dalloc(std::size_t bytes, const char* what) {
+    void* p = nullptr;
+    hip_ok(hipMalloc(&p, bytes ? bytes : 1), what);
+    dev_.push_back(p);
+    return p;
+}
+
+// Precision 32: per run two theta fields, its plan, cruise flags and
+// max_steps per-step records; the batch of all runs is built by
+// rebuild_f32.  Called from the constructor with the set-up lock held.
+void Ensemble::setup_f32() {
+    const std::size_t nn = runs_.size();
+    const std::size_t N = static_cast<std::size_t>(nx_) * ny_;
+    ld_ = (nx_ + 3) & ~3;
+    const std::size_t F = static_cast<std::size_t>(ld_) * ny_ * sizeof(float);
+    const hipStream_t s = S(stream_);
+    d_stage_ = static_cast<double*>(dalloc(N * sizeof(double), "staging"));
+    d_stats_ = static_cast<StepStats*>(
+        dalloc(nn * static_cast<std::size_t>(std::max(max_steps_, 1)) * sizeof(StepStats),
+               "per-step records"));
+    d_jobs_ = dalloc(nn * sizeof(lpbf_f32_col_job), "column jobs");
+    std::vector<double> init(N);
+    cruise_.resize(nn);
+    fcur_.assign(nn, 0);
+    for (std::size_t m = 0; m < nn; ++m) {
+        EnsembleMember const& r = runs_[m];
+        const std::size_t ns = r.plan.size();
+        tref_.push_back(opt_.T_ref < 0.0 ? r.Tamb : opt_.T_ref);
+        for (int q = 0; q < 2; ++q) {
+            float* f = static_cast<float*>(dalloc(F, "field"));
+            hip_ok(hipMemsetAsync(f, 0, F, s), "field clear");
+            fbuf_.push_back(f);
+        }
+        const std::vector<LaserTick> lt = to_laser(r.plan);
+        cruise_[m].resize(ns);
+        for (std::size_t i = 0; i < ns; ++i) cruise_[m][i] = met_[m].cruise(r.plan[i]) ? 1 : 0;
+        d_plan_.push_back(static_cast<LaserTick*>(dalloc(ns * sizeof(LaserTick), "plan")));
+        d_cruise_.push_back(static_cast<unsigned char*>(dalloc(ns, "cruise flags")));
+        if (ns) {
+            hip_ok(hipMemcpyAsync(d_plan_.back(), lt.data(), ns * sizeof(LaserTick),
+                                  hipMemcpyHostToDevice, s), "plan upload");
+            hip_ok(hipMemcpyAsync(d_cruise_.back(), cruise_[m].data(), ns, hipMemcpyHostToDevice,
+                                  s), "cruise upload");
+        }
+        std::fill(init.begin(), init.end(), r.Tamb);
+        hip_ok(hipMemcpyAsync(d_stage_, init.data(), N * sizeof(double), hipMemcpyHostToDevice, s),
+               "initial field");
+        lpbf_launch_T_to_theta(d_stage_, nx_, ny_, tref_[m], fbuf_[2 * m], ld_, s);
+        // the host buffers are reused for the next run
+        hip_ok(hipStreamSynchronize(s), "initial field");
+        lib_ok("initial field");
+    }
+    for (std::size_t m = 0; m < nn; ++m)
+        if (!runs_[m].plan.empty()) active_.push_back(static_cast<int>(m));
+    rebuild_f32(0);
+}
+
+// Buffer (0/1) of run m's pair holding its latest field: the launches since
+// the last rebuild moved every active run's field once each.
+int Ensemble::latest_f32(int m) const {
+    const int c = fcur_[static_cast<std::size_t>(m)];
+    return std::find(active_.begin(), active_.end(), m) != active_.end() ? c ^ (flaunch_ & 1) : c;
+}
+
+// (Re)builds the batch over the runs still running at step `next` (the
+// first step not yet queued); the queued work is finished first, so the
+// job table and the batch's device arguments are rewritten while idle.
+// Called with the set-up lock held.
+void Ensemble::rebuild_f32(int next) {
+    fresh_ = true;
+    const hipStream_t s = S(stream_);
+    hip_ok(hipStreamSynchronize(s), "rebuild sync");
+    for (int m : active_) fcur_[static_cast<std::size_t>(m)] ^= flaunch_ & 1;
+    flaunch_ = 0;
+    std::vector<int> keep;
+    for (int m : active_)
+        if (n_steps(m) > next) keep.push_back(m);
+    active_ = keep;
+    if (fbatch_) lpbf_f32_batch_destroy(FB(fbatch_));
+    fbatch_ = nullptr;
+    if (active_.empty()) return;
+    const std::size_t na = active_.size();
+    std::vector<lpbf_f32_instance> inst(na);
+    std::vector<lpbf_f32_col_job> jobs(na);
+    for (std::size_t a = 0; a < na; ++a) {
+        const int m = active_[a];
+        const std::size_t um = static_cast<std::size_t>(m);
+        const int c = fcur_[um];
+        lpbf_f32_instance& I = inst[a];
+        I = lpbf_f32_instance{};
+        I.k = runs_[um].k;
+        I.theta0 = fbuf_[2 * um + static_cast<std::size_t>(c)];
+        I.theta1 = fbuf_[2 * um + static_cast<std::size_t>(c ^ 1)];
+        I.ld = ld_;
+        I.size_class = 0;
+        I.T_ref = tref_[um];
+        I.T_bc = opt_.T_bc;
+        I.Tm = Tm_;
+        I.d_plan = d_plan_[um];
+        I.d_stats = d_stats_ + um * static_cast<std::size_t>(max_steps_);
+        I.n_steps = n_steps(m);
+        lpbf_f32_col_job& J = jobs[a];
+        J = lpbf_f32_col_job{};
+        J.theta[0] = I.theta0;
+        J.theta[1] = I.theta1;
+        J.stats = I.d_stats;
+        J.cruise = d_cruise_[um];
+        J.col = d_cols_ + um * static_cast<std::size_t>(ny_);
+        J.T_ref = I.T_ref;
+        J.ld = ld_;
+        J.nx = nx_;
+        J.ny = ny_;
+        J.n_steps = I.n_steps;
+    }
+    lpbf_f32_batch* b = nullptr;
+    rc_ok(lpbf_f32_batch_create(inst.data(), static_cast<int>(na), opt_.bc, &b), "batch create");
+    fbatch_ = b;
+    rc_ok(lpbf_f32_batch_set_tiling(b, 0, opt_.depth, opt_.rows), "batch tiling");
+    hip_ok(hipMemcpy(d_jobs_, jobs.data(), na * sizeof(lpbf_f32_col_job), hipMemcpyHostToDevice),
+           "column jobs");
+}
+
 // Teardown cannot report failures; a sticky device error stays visible to
 // the next HIP call of the caller.  Called with the set-up lock held.
 void Ensemble::release() noexcept {
     if (stream_) (void)hipStreamSynchronize(S(stream_));
+    if (fbatch_) lpbf_f32_batch_destroy(FB(fbatch_));
+    fbatch_ = nullptr;
+    fbuf_.clear();
+    d_stats_ = nullptr; d_jobs_ = nullptr; d_stage_ = nullptr;
     if (gexec_) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(gexec_));
     if (graph_) (void)hipGraphDestroy(static_cast<hipGraph_t>(graph_));
     gexec_ = graph_ = nullptr;
@@ -169,21 +313,40 @@ void Ensemble::release() noexcept {
 
 void Ensemble::upload(int m, const double* T) {
     if (!T || m < 0 || m >= size()) throw std::invalid_argument("Ensemble::upload: bad run or field");
-    const int cur = std::min(done_, n_steps(m)) & 1;
     const std::size_t N = static_cast<std::size_t>(nx_) * ny_;
-    hip_ok(hipMemcpyAsync(buf_[2 * static_cast<std::size_t>(m) + cur], T, N * sizeof(double),
-                          hipMemcpyHostToDevice, S(stream_)), "upload");
+    const std::size_t um = static_cast<std::size_t>(m);
+    if (opt_.precision == 32) {
+        hip_ok(hipMemcpyAsync(d_stage_, T, N * sizeof(double), hipMemcpyHostToDevice, S(stream_)),
+               "upload");
+        lpbf_launch_T_to_theta(d_stage_, nx_, ny_, tref_[um],
+                               fbuf_[2 * um + static_cast<std::size_t>(latest_f32(m))], ld_,
+                               S(stream_));
+    } else {
+        const int cur = std::min(done_, n_steps(m)) & 1;
+        hip_ok(hipMemcpyAsync(buf_[2 * um + cur], T, N * sizeof(double), hipMemcpyHostToDevice,
+                              S(stream_)), "upload");
+    }
     hip_ok(hipStreamSynchronize(S(stream_)), "upload sync");
+    lib_ok("upload");
 }
 
 void Ensemble::download(int m, double* T) const {
     if (!T || m < 0 || m >= size())
         throw std::invalid_argument("Ensemble::download: bad run or field");
-    const int cur = std::min(done_, n_steps(m)) & 1;
     const std::size_t N = static_cast<std::size_t>(nx_) * ny_;
-    hip_ok(hipMemcpyAsync(T, buf_[2 * static_cast<std::size_t>(m) + cur], N * sizeof(double),
-                          hipMemcpyDeviceToHost, S(stream_)), "download");
+    const std::size_t um = static_cast<std::size_t>(m);
+    if (opt_.precision == 32) {
+        lpbf_launch_theta_to_T(fbuf_[2 * um + static_cast<std::size_t>(latest_f32(m))], ld_, nx_,
+                               ny_, tref_[um], d_stage_, S(stream_));
+        hip_ok(hipMemcpyAsync(T, d_stage_, N * sizeof(double), hipMemcpyDeviceToHost,
+                              S(stream_)), "download");
+    } else {
+        const int cur = std::min(done_, n_steps(m)) & 1;
+        hip_ok(hipMemcpyAsync(T, buf_[2 * um + cur], N * sizeof(double), hipMemcpyDeviceToHost,
+                              S(stream_)), "download");
+    }
     hip_ok(hipStreamSynchronize(S(stream_)), "download sync");
+    lib_ok("download");
 }
 
 // Batches are pipelined: batch b+1 is queued (launches, ring and column
@@ -241,11 +404,84 @@ void Ensemble::capture() {
     gexec_ = x;
 }
 
-Ensemble::Pending Ensemble::enqueue(int s0, int bn, int slot) {
+// Precision 32: launches of up to `depth` steps that end at the batch end,
+// at any active run's last step (then the batch is rebuilt without it) and,
+// with cruise_cols, at any active run's cruise step; after a launch whose
+// last step is some run's cruise step, the hot columns of every active run
+// (each job copies only on its own cruise steps).  Then the records of
+// the batch steps for all runs and the columns go to the pinned half `slot`.
+Ensemble::Pending Ensemble::enqueue_f32(int s0, int bn, int slot) {
     const hipStream_t s = S(stream_);
     const int n = size();
     const std::size_t B = static_cast<std::size_t>(opt_.batch);
-    Pending p{s0, bn, slot, bn, -1};
+    Pending p{s0, bn, slot, bn, -1, std::vector<char>(static_cast<std::size_t>(bn), 0)};
+    auto ends_at = [&](int st) {
+        for (int m : active_)
+            if (n_steps(m) == st + 1) return true;
+        return false;
+    };
+    auto cruise_at = [&](int st) {
+        for (int m : active_)
+            if (st < n_steps(m) && cruise_[static_cast<std::size_t>(m)][static_cast<std::size_t>(st)])
+                return true;
+        return false;
+    };
+    int i = 0;
+    while (i < bn && fbatch_) {
+        const int limit = std::min(opt_.depth, bn - i);
+        int L = 1;
+        while (L < limit) {
+            const int st = s0 + i + L - 1;
+            if (ends_at(st) || (opt_.cruise_cols && cruise_at(st))) break;
+            ++L;
+        }
+        if (fresh_) {   // the batch's first launch allocates its device arguments
+            std::lock_guard<std::mutex> lock(setup_mutex());
+            rc_ok(lpbf_f32_batch_enqueue_launch(FB(fbatch_), s0 + i, L, s), "step launch");
+            fresh_ = false;
+        } else {
+            rc_ok(lpbf_f32_batch_enqueue_launch(FB(fbatch_), s0 + i, L, s), "step launch");
+        }
+        ++flaunch_;
+        const int j = i + L - 1, st = s0 + j;
+        p.col_at[static_cast<std::size_t>(j)] = 1;
+        if (cruise_at(st)) {
+            rc_ok(lpbf_launch_hot_cols_f32(static_cast<const lpbf_f32_col_job*>(d_jobs_),
+                                           static_cast<int>(active_.size()), flaunch_ & 1, st, j,
+                                           static_cast<long long>(n) * ny_, s),
+                  "hot columns");
+            p.c_lo = std::min(p.c_lo, j);
+            p.c_hi = std::max(p.c_hi, j);
+        }
+        i += L;
+        if (ends_at(st)) {
+            std::lock_guard<std::mutex> lock(setup_mutex());
+            rebuild_f32(st + 1);
+        }
+    }
+    hip_ok(hipMemcpy2DAsync(h_ring_ + static_cast<std::size_t>(slot) * n * B,
+                            B * sizeof(StepStats), d_stats_ + s0,
+                            static_cast<std::size_t>(max_steps_) * sizeof(StepStats),
+                            static_cast<std::size_t>(bn) * sizeof(StepStats),
+                            static_cast<std::size_t>(n), hipMemcpyDeviceToHost, s),
+           "record copy");
+    const std::size_t step_cols = static_cast<std::size_t>(n) * ny_;
+    double* hc = h_cols_ + slot * B * step_cols;
+    if (p.c_hi >= p.c_lo)
+        hip_ok(hipMemcpyAsync(hc + p.c_lo * step_cols, d_cols_ + p.c_lo * step_cols,
+                              static_cast<std::size_t>(p.c_hi - p.c_lo + 1) * step_cols *
+                                  sizeof(double),
+                              hipMemcpyDeviceToHost, s), "column copy");
+    hip_ok(hipEventRecord(static_cast<hipEvent_t>(done_ev_[slot]), s), "batch event");
+    return p;
+}
+
+Ensemble::Pending Ensemble::enqueue(int s0, int bn, int slot) {
+    if (opt_.precision == 32) return enqueue_f32(s0, bn, slot);
+    const hipStream_t s = S(stream_);
+    const int n = size();
+    const std::size_t B = static_cast<std::size_t>(opt_.batch);
+    Pending p{s0, bn, slot, bn, -1, {}};
 
     // batch steps that copy a hot column for some run: [c_lo, c_hi]
     for (int m = 0; m < n; ++m) {
@@ -295,7 +531,9 @@ void Ensemble::complete(Pending const& p) {
         const int end = std::min(p.bn, n_steps(m) - p.s0);
         for (int i = 0; i < end; ++i) {
             Tick const& tk = at(m).plan[static_cast<std::size_t>(p.s0 + i)];
-            const double* col = met.cruise(tk)
+            const bool take = met.cruise(tk) &&
+                              (opt_.precision == 64 || p.col_at[static_cast<std::size_t>(i)]);
+            const double* col = take
                                     ? hc + static_cast<std::size_t>(i) * step_cols +
                                           static_cast<std::size_t>(m) * ny_
                                     : nullptr;
@@ -341,7 +579,7 @@ int lpbf_ensemble_create(const StepConst* k, int n_runs, int phys, double Tm, do
         total += n_steps[m];
     }
     if (total > 0 && !plan) return bad_arg();
-    if (opt && opt->precision != 64) return bad_arg();
+    if (opt && opt->precision != 64 && opt->precision != 32) return bad_arg();
     return guarded([&] {
         lpbf::Domain dom;
         dom.nx = k[0].nx;
@@ -375,6 +613,10 @@ int lpbf_ensemble_create(const StepConst* k, int n_runs, int phys, double Tm, do
             o.rows = opt->rows;
             o.keep_records = opt->keep_records != 0;
             o.graph = opt->graph != 0;
+            o.precision = opt->precision;
+            o.depth = opt->depth;
+            o.T_ref = opt->T_ref;
+            o.cruise_cols = opt->cruise_cols != 0;
         }
         *out = new lpbf_ensemble(dom, phys, Tm, std::move(runs), o);
     });

@@ -52,6 +52,19 @@ __global__ __launch_bounds__(kThreads) void k_stats_finalize_multi(
                       I.stats + step0 + j);
 }
 
+// Block m: the hot column of job m after `step` (lpbf_launch_hot_cols_f32).
+__global__ __launch_bounds__(kThreads) void k_hot_cols_f32(const lpbf_f32_col_job* __restrict__ jobs,
+                                                           int q, int step, int slot,
+                                                           long long col_pitch) {
+    const lpbf_f32_col_job& J = jobs[blockIdx.x];
+    if (step >= J.n_steps || !J.cruise[step]) return;
+    const int xh = static_cast<int>(J.stats[step].arg % J.nx);
+    const float* f = J.theta[q];
+    double* col = J.col + static_cast<long long>(slot) * col_pitch;
+    for (int y = threadIdx.x; y < J.ny; y += kThreads)
+        col[y] = J.T_ref + static_cast<double>(f[static_cast<long long>(y) * J.ld + xh]);
+}
+
 template <int K, int BC>
 void launch_kb(bool rad, const InstArgs* inst, const int* wg_inst, int total_wg, int parity,
                int step0, hipStream_t s) {
@@ -411,6 +424,30 @@ int lpbf_f32_batch_launch_graph(lpbf_f32_batch* b, hipStream_t stream) {
     for (size_t q = 0; q < B.groups.size(); ++q)
         if (B.groups[q].p != B.graph_parity[q]) return fail(LPBF_ERR_INVALID_ARGUMENT);
     return guarded([&] { lpbf_amd::ok(hipGraphLaunch(b->b.exec, stream)); });
+}
+
+int lpbf_f32_batch_enqueue_launch(lpbf_f32_batch* b, int step0, int n_steps,
+                                  hipStream_t stream) {
+    if (!b || step0 < 0 || n_steps < 1 || n_steps > lpbf_amd::kMaxDepth ||
+        !b->b.plans_cover(step0, n_steps))
+        return fail(LPBF_ERR_INVALID_ARGUMENT);
+    for (auto const& I : b->b.inst)
+        if (n_steps > b->b.depth_of(I.size_class)) return fail(LPBF_ERR_INVALID_ARGUMENT);
+    return guarded([&] {
+        b->b.ensure_groups();
+        for (auto& g : b->b.groups) b->b.launch(g, n_steps, step0, stream);
+    });
+}
+
+int lpbf_launch_hot_cols_f32(const lpbf_f32_col_job* d_jobs, int n_jobs, int q, int step,
+                             int slot, long long col_pitch, hipStream_t stream) {
+    if (!d_jobs || n_jobs < 0 || (q != 0 && q != 1) || step < 0 || slot < 0 || col_pitch < 0)
+        return fail(LPBF_ERR_INVALID_ARGUMENT);
+    if (n_jobs == 0) return 0;
+    hipLaunchKernelGGL(lpbf_amd::k_hot_cols_f32, dim3(n_jobs), dim3(lpbf_amd::kThreads), 0, stream,
+                       d_jobs, q, step, slot, col_pitch);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : fail(static_cast<int>(e));
 }
 
 int lpbf_f32_batch_current(const lpbf_f32_batch* b, int instance) {

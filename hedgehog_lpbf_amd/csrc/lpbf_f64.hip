@@ -68,10 +68,71 @@ struct Region {
     int ring_len;
     const unsigned char* cruise;
     double* cols;
+    Recips rc;      // the launch's quotients, computed on the host (recips(k))
 };
 
 // One run of an ensemble (include/lpbf_kernels.h, lpbf_ensemble_run).
 using EnsMember = lpbf_ensemble_run;
+
+// Wavefront reductions on DPP lane moves and lane reads instead of LDS
+// permutes: within each 16-lane row, partners by quad_perm [1,0,3,2], then
+// [2,3,0,1], then the half-row and row mirrors (both lanes of a pair compute
+// a+b from the same two values, so every lane of a row ends with the same
+// bits); then the four row results, read from lanes 0, 16, 32, 48, in a
+// fixed order.  The result is the same in every lane, and the order does not
+// depend on the launch, so the FP64 sums are reproducible.
+template <int kCtrl>
+__device__ __forceinline__ int dpp_i32(int v) {
+    return __builtin_amdgcn_mov_dpp(v, kCtrl, 0xF, 0xF, false);
+}
+template <int kCtrl>
+__device__ __forceinline__ long long dpp_i64(long long v) {
+    const int lo = dpp_i32<kCtrl>(static_cast<int>(v));
+    const int hi = dpp_i32<kCtrl>(static_cast<int>(v >> 32));
+    return (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo);
+}
+template <int kCtrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+    return __longlong_as_double(dpp_i64<kCtrl>(__double_as_longlong(v)));
+}
+__device__ __forceinline__ long long lane_i64(long long v, int l) {
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(v), l);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), l);
+    return (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo);
+}
+__device__ __forceinline__ double lane_f64(double v, int l) {
+    return __longlong_as_double(lane_i64(__double_as_longlong(v), l));
+}
+
+__device__ __forceinline__ double wave_total(double v) {
+    v = v + dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
+    v = v + dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
+    v = v + dpp_f64<0x141>(v);   // row_half_mirror
+    v = v + dpp_f64<0x140>(v);   // row_mirror
+    return (lane_f64(v, 0) + lane_f64(v, 16)) + (lane_f64(v, 32) + lane_f64(v, 48));
+}
+
+// Largest value with the smallest index on ties (exact, so any pairing
+// gives the serial result).
+__device__ __forceinline__ void take_hot(double& t, long long& a, double t2, long long a2) {
+    if (t2 > t || (t2 == t && a2 < a)) { t = t2; a = a2; }
+}
+template <int kCtrl>
+__device__ __forceinline__ void hot_step(double& t, long long& a) {
+    take_hot(t, a, dpp_f64<kCtrl>(t), dpp_i64<kCtrl>(a));
+}
+__device__ __forceinline__ void wave_hot(double& t, long long& a) {
+    hot_step<0xB1>(t, a);
+    hot_step<0x4E>(t, a);
+    hot_step<0x141>(t, a);
+    hot_step<0x140>(t, a);
+    double rt = lane_f64(t, 0);
+    long long ra = lane_i64(a, 0);
+#pragma unroll
+    for (int r = 16; r < kWave; r += 16) take_hot(rt, ra, lane_f64(t, r), lane_i64(a, r));
+    t = rt;
+    a = ra;
+}
 
 // Finishes step j of a batch: merges its nblk block records (in a fixed
 // order, so the sums are reproducible) into ring[j] and, on cruise steps,
@@ -127,22 +188,16 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
     }
     if (RED == RED_BATCH && blockIdx.x == gridDim.x - 1) {
         // the extra workgroup finishes the previous step of the batch while
-        // the others compute this one: its records are in the other half of
-        // the record buffer and its output is this launch's input
-        const int j = red_fill, nblk = ncb;
+        // the others compute this one: its records (one per wavefront) are
+        // in the other half of the record buffer and its output is this
+        // launch's input
+        const int j = red_fill, nblk = ncb * (kThreads / kWave);
         if (j > 0)
             finish_batch_step(recs + ((j - 1) & 1) * nblk, nblk, g.nx, g.ny, j - 1, g.in,
                               g.cruise, g.ring, g.cols, g.ny);
         return;
     }
-    if (RED == RED_BATCH) recs += (red_fill & 1) * ncb;
-    if (g.plan) {
-        const LaserTick t =
-            g.plan[(RED == RED_BATCH || RED == RED_ENSEMBLE) ? tick : *g.step];
-        k.x_l = t.x_l;
-        k.y_l = t.y_l;
-        k.q_scale = t.q_scale;
-    }
+    if (RED == RED_BATCH) recs += (red_fill & 1) * ncb * (kThreads / kWave);
     const int lane = threadIdx.x & (kWave - 1);
     // wave-uniform: the block, its rows and the row loop live in SGPRs
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -159,8 +214,8 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
     int bpos = -1, xs0 = 0, xs1 = 0;
 
     if (task < g.ntasks) {
-        const Recips rcp_k = recips(k);
-        const Beam bm = beam_of(k);
+        // the ensemble form takes each run's constants from memory
+        const Recips rcp_k = RED == RED_ENSEMBLE ? recips(k) : g.rc;
         const int tx = task % g.nwx, ty = task / g.nwx;
         const int c0 = tx * kBlockCols;
         const int ya = ty * g.R;
@@ -170,7 +225,6 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
         xs0 = x0;
         xs1 = x1;
         const int ecol = (lane == 0) ? c0 - 1 : c0 + kBlockCols;
-        const bool eok = (lane == 0 && c0 > 0) || (lane == kWave - 1 && c0 + kBlockCols < nx);
         const bool tile = (g.bc == BC_TILE);
 
         // rows through global-address-space pointers: the ensemble form reads
@@ -180,15 +234,22 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
         // the row prefetches)
         const gdouble* gin = (const gdouble*)g.in;
         gdouble* gout = (gdouble*)g.out;
+        // every row load is unconditional, at a column clamped into the
+        // row: a load under a divergent lane condition is waited for before
+        // the next one is issued, which serialised the first rows' round
+        // trips (the bulk of a small plate's step).  Lanes past the last
+        // column compute nothing, and the edge ghost (ce) is used only where
+        // it is a real column, so the clamped values are never used.  The
+        // row above the first row of a mirror-edged field is that row again
+        // (the ghost equals the centre), read rather than copied so that no
+        // load waits for another.
+        const int lx0 = min(x0, nx - 1), lx1 = min(x1, nx - 1);
+        const int lxe = min(max(ecol, 0), nx - 1);
         const gdouble* rc = gin + static_cast<long long>(ya) * nx;
-        double c0v = a0 ? rc[x0] : 0.0, c1v = a1 ? rc[x1] : 0.0;
-        double ce = eok ? rc[ecol] : 0.0;
-        double p0v = c0v, p1v = c1v;
-        if (ya > 0 || tile) {
-            const gdouble* rp = gin + static_cast<long long>(ya - 1) * nx;
-            p0v = a0 ? rp[x0] : 0.0;
-            p1v = a1 ? rp[x1] : 0.0;
-        }
+        double c0v = rc[lx0], c1v = rc[lx1];
+        double ce = rc[lxe];
+        const gdouble* rp = gin + static_cast<long long>((ya > 0 || tile) ? ya - 1 : ya) * nx;
+        double p0v = rp[lx0], p1v = rp[lx1];
         if (tile) {  // halo rows pass through unchanged (step_task.hpp:78-84)
             if (ya == 0) {
                 const long long o = -static_cast<long long>(nx);
@@ -214,11 +275,21 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
             q0[d] = q1[d] = qe[d] = 0.0;
             if (r <= r_last) {
                 const gdouble* rn = gin + static_cast<long long>(r) * nx;
-                q0[d] = a0 ? rn[x0] : 0.0;
-                q1[d] = a1 ? rn[x1] : 0.0;
-                qe[d] = eok ? rn[ecol] : 0.0;
+                q0[d] = rn[lx0];
+                q1[d] = rn[lx1];
+                qe[d] = rn[lxe];
             }
         }
+        // the laser of a plan-driven step, read after the rows are in
+        // flight so that the two round trips overlap
+        if (g.plan) {
+            const LaserTick t =
+                g.plan[(RED == RED_BATCH || RED == RED_ENSEMBLE) ? tick : *g.step];
+            k.x_l = t.x_l;
+            k.y_l = t.y_l;
+            k.q_scale = t.q_scale;
+        }
+        const Beam bm = beam_of(k);
         for (int y = ya; y < yb; ++y) {
             const bool has_next = (y + 1 < ny || tile);
             const double n0v = has_next ? q0[0] : c0v, n1v = has_next ? q1[0] : c1v;
@@ -232,9 +303,9 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
             const int r = y + 1 + kAhead;
             if (r <= r_last) {
                 const gdouble* rn = gin + static_cast<long long>(r) * nx;
-                q0[kAhead - 1] = a0 ? rn[x0] : 0.0;
-                q1[kAhead - 1] = a1 ? rn[x1] : 0.0;
-                qe[kAhead - 1] = eok ? rn[ecol] : 0.0;
+                q0[kAhead - 1] = rn[lx0];
+                q1[kAhead - 1] = rn[lx1];
+                qe[kAhead - 1] = rn[lxe];
             }
             const double up0 = lane_from_left(c0v), up1 = lane_from_left(c1v);
             const double dn0 = lane_from_right(c0v), dn1 = lane_from_right(c1v);
@@ -247,6 +318,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
             const double ypos = static_cast<double>(yg) * k.dy;
             const bool far = beam_far(ypos, bm, k);
             gdouble* orow = gout + static_cast<long long>(y) * nx;
+            bool mol[2] = {false, false};
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 const int x = j ? x1 : x0;
@@ -274,11 +346,7 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                         pa.tmax = v;
                         bpos = 2 * y + j;
                     }
-                    if (v > g.Tm) {
-                        if (yg < pa.ymin) pa.ymin = yg;
-                        if (yg > pa.ymax) pa.ymax = yg;
-                        ++molten;
-                    }
+                    mol[j] = v > g.Tm;
                     if (RED == RED_STATS || RED == RED_RING || RED == RED_BATCH ||
                         RED == RED_ENSEMBLE) {
                         // held Dirichlet cells have src = lossv = 0 and add
@@ -287,6 +355,16 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
                         s_src += src;
                         s_loss += lossv;
                     }
+                }
+            }
+            if (RED != RED_NONE) {
+                // molten cells and rows per wavefront from ballots: wave-
+                // uniform, so they need no cross-lane reduction
+                const unsigned long long b0 = __ballot(mol[0]), b1 = __ballot(mol[1]);
+                molten += __popcll(b0) + __popcll(b1);
+                if (b0 | b1) {
+                    pa.ymin = min(pa.ymin, yg);
+                    pa.ymax = max(pa.ymax, yg);
                 }
             }
             p0v = c0v; p1v = c1v;
@@ -300,14 +378,21 @@ __global__ __launch_bounds__(kThreads) void k_step_f64(Region g, StepConst k,
     __shared__ PartialAcc s_pa[kThreads / kWave];
     __shared__ double s_sum[3][kThreads / kWave];
     __shared__ long long s_mol[kThreads / kWave];
-    const PartialAcc w = wave_reduce(pa);
+    PartialAcc w = pa;   // ymin, ymax: already the wavefront's
+    wave_hot(w.tmax, w.arg);
     double w_dT = 0.0, w_src = 0.0, w_loss = 0.0;
-    long long w_mol = 0;
+    const long long w_mol = molten;
     if (RED == RED_STATS || RED == RED_RING || RED == RED_BATCH || RED == RED_ENSEMBLE) {
-        w_dT = wave_sum(s_dT);
-        w_src = wave_sum(s_src);
-        w_loss = wave_sum(s_loss);
-        w_mol = wave_sum(molten);
+        w_dT = wave_total(s_dT);
+        w_src = wave_total(s_src);
+        w_loss = wave_total(s_loss);
+    }
+    if (RED == RED_BATCH) {
+        // one record per wavefront, written without a workgroup merge
+        if (lane == 0)
+            recs[bid * (kThreads / kWave) + wave] =
+                StepStats{w.tmax, w.arg, w.ymin, w.ymax, w_mol, w_dT, w_src, w_loss};
+        return;
     }
     if (lane == 0) {
         s_pa[wave] = w;
@@ -515,10 +600,12 @@ void launch_step(const Region& g, const StepConst& k, int phys, void* red, int f
     // previous step; the ensemble form repeats the grid for every run
     const int per = grid_of(g) + ((RED == RED_BATCH || RED == RED_ENSEMBLE) ? 1 : 0);
     const dim3 grid(per * n_runs), block(kThreads);
+    Region gr = g;
+    gr.rc = recips(k);
     if (phys == LPBF_PHYS_EXTENDED)
-        hipLaunchKernelGGL((k_step_f64<LPBF_PHYS_EXTENDED, RED>), grid, block, 0, s, g, k, red, fill);
+        hipLaunchKernelGGL((k_step_f64<LPBF_PHYS_EXTENDED, RED>), grid, block, 0, s, gr, k, red, fill);
     else
-        hipLaunchKernelGGL((k_step_f64<LPBF_PHYS_BASELINE, RED>), grid, block, 0, s, g, k, red, fill);
+        hipLaunchKernelGGL((k_step_f64<LPBF_PHYS_BASELINE, RED>), grid, block, 0, s, gr, k, red, fill);
     check_launch();
 }
 
@@ -677,7 +764,10 @@ void lpbf_launch_step_ex_plan(const double* Tin, double* Tout, const StepConst* 
     launch_step<RED_RING>(g, *k, phys, block_stats, 0, stream);
 }
 
-int lpbf_step_ex_batch_nblocks(int nx, int ny) { return lpbf_step_ex_plan_nblocks(nx, ny); }
+int lpbf_step_ex_batch_nblocks(int nx, int ny) {
+    // one record per wavefront
+    return lpbf_step_ex_plan_nblocks(nx, ny) * (kThreads / kWave);
+}
 
 void lpbf_launch_step_ex_batch(const double* Tin, double* Tout, const StepConst* k, int phys,
                                int bc, double T_bc, double Tm, const LaserTick* d_ticks, int j,

@@ -75,14 +75,17 @@ struct Beam {
 };
 __device__ __forceinline__ Beam beam_of(const StepConst& k) { return Beam{k.x_l, k.y_l, k.q_scale}; }
 
-// Reciprocals of the divisors that are constant over a launch.
+// Quotients of launch constants (IEEE divisions, so the host and the device
+// compute the same bits): three reciprocals of divisors and the substrate
+// conductance K / dSub of the baseline loss.
 struct Recips {
     double rho_cp;      // 1 / rho_cp
     double rho_cp_h;    // 1 / (rho_cp * h)
     double dSub;        // 1 / dSub
+    double K_dSub;      // K / dSub
 };
-__device__ __forceinline__ Recips recips(const StepConst& k) {
-    return Recips{1.0 / k.rho_cp, 1.0 / (k.rho_cp * k.h), 1.0 / k.dSub};
+__host__ __device__ __forceinline__ Recips recips(const StepConst& k) {
+    return Recips{1.0 / k.rho_cp, 1.0 / (k.rho_cp * k.h), 1.0 / k.dSub, k.K / k.dSub};
 }
 
 // True when every cell of the row at ypos has a Gaussian exponent below
@@ -118,7 +121,7 @@ __device__ __forceinline__ double cell_base(double T, double Txm, double Txp,
         const double Ta4 = k.Tamb * k.Tamb * k.Tamb * k.Tamb;
         q_top += k.eps * kSigmaSB * (T4 - Ta4);
     }
-    const double q_bot = (k.K / k.dSub) * (T - k.Tsub);
+    const double q_bot = rc.K_dSub * (T - k.Tsub);
     const double loss = div_rn((q_top + q_bot) * k.inv_h, k.rho_cp, rc.rho_cp);
     const double dTdt = k.alpha * (lap_x + lap_y) + S - loss;
     src = k.dt * S;

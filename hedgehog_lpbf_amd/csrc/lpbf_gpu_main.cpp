@@ -8,8 +8,9 @@
 //   --quiet      one RESULT line (:201-210, 321-331);
 //   --manifest   a CSV of runs, one RESULT line each, in one process (:234-290);
 //                --jobs=N runs N rows at once (one stream each), --ensemble
-//                steps the rows of each physics together (lpbf::Ensemble, one
-//                launch per step for all of them).
+//                steps the rows of each physics together (lpbf::Ensemble: in
+//                FP64 one launch per step for all of them, in FP32 the
+//                batched kernel, up to --depth steps per launch).
 //
 // The field stays on the GPU for the whole run (lpbf::DeviceRun); the host
 // synchronises once per --batch steps.  --precision=32 selects the FP32
@@ -95,8 +96,10 @@ struct CLI {
            "  --manifest=FILE      loop a runs CSV in-process\n"
            "  --jobs=N             run N manifest rows at once, each on its own stream\n"
            "                       (RESULT lines still in manifest order)\n"
-           "  --ensemble           step all manifest rows of one physics together (FP64),\n"
-           "                       one launch per step; wall_ms is the group's\n";
+           "  --ensemble           step all manifest rows of one physics together: FP64\n"
+           "                       one launch per step, FP32 (--precision=32, baseline\n"
+           "                       physics) up to --depth steps per launch; wall_ms is\n"
+           "                       the group's\n";
     std::exit(rc);
 }
 
@@ -290,6 +293,12 @@ std::vector<RowResult> run_rows_ensemble(std::vector<RunSpec> const& rows, CLI c
         o.bc = ro.bc;
         o.T_bc = ro.T_bc;
         o.batch = ro.batch;
+        o.precision = ro.precision;
+        if (ro.precision == 32) {   // the FP32 kernel's tiling; the FP64 launch picks its own
+            o.depth = ro.depth;
+            o.rows = ro.rows;
+            o.cruise_cols = ro.cruise_cols;
+        }
         Ensemble ens(dom, phys, Tm, std::move(members), o);
         const auto w0 = std::chrono::steady_clock::now();
         ens.run();
@@ -363,10 +372,6 @@ int run_manifest(CLI const& cli) {
         rows.push_back(rs);
     }
     if (cli.ensemble) {
-        if (cli.precision != 64) {
-            std::cerr << "--ensemble steps the FP64 kernels (--precision=64)\n";
-            return 1;
-        }
         const std::vector<RowResult> res = run_rows_ensemble(rows, cli);
         for (std::size_t i = 0; i < rows.size(); ++i) emit_result_line(rows[i], res[i]);
     } else if (cli.jobs <= 1) {

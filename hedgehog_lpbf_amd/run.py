@@ -5,8 +5,9 @@ DeviceRun mirrors lpbf::DeviceRun: the field stays on the GPU, steps run in
 batches with one host synchronisation per batch, and per-step records
 (statistics, W, W_sub) plus the running maxima of the reference MetricsTask
 (include/gpu_step_task.hpp:375-404) come back.  Ensemble mirrors
-lpbf::Ensemble: many FP64 runs of one plate (the reference manifest sweep,
-src/gpu/hh_gpu_main.cpp:234-290) stepped by one launch per step for all.
+lpbf::Ensemble: many runs of one plate (the reference manifest sweep,
+src/gpu/hh_gpu_main.cpp:234-290) stepped together, FP64 by one launch per
+step for all, FP32 by the batched multi-instance kernel.
 """
 from __future__ import annotations
 
@@ -109,14 +110,17 @@ class DeviceRun:
 
 
 class Ensemble:
-    """n FP64 runs of one plate, advanced together (lpbf_ensemble_*).
+    """n runs of one plate, advanced together (lpbf_ensemble_*): FP64 (one
+    ensemble launch per step) or, with precision=32, the batched FP32 kernel
+    (up to `depth` steps per launch for all runs still running).
 
     runs: a list of dicts with keys k (StepConst), plan ((n_i, 3) laser
     ticks), and optionally x_csv, wrapped (cruise gate) and Tamb."""
 
     def __init__(self, runs, *, Tm: float, Lx: float, Ly: float, phys: int = 0, bc: int = 0,
                  T_bc: float = 300.0, batch: int = 64, rows: int = 0, keep_records: bool = True,
-                 graph: bool = True):
+                 graph: bool = True, precision: int = 64, depth: int = 4, T_ref: float = -1.0,
+                 cruise_cols: bool = False):
         n = len(runs)
         ks = (StepConst * n)(*[r["k"] for r in runs])
         lens = [int(np.asarray(r["plan"]).reshape(-1, 3).shape[0]) for r in runs]
@@ -132,8 +136,8 @@ class Ensemble:
         tamb = np.ascontiguousarray([float(r.get("Tamb", 300.0)) for r in runs])
         steps = np.ascontiguousarray(lens, dtype=np.int32)
         self._keep = [ks, plan, xc, wr, tamb, steps]
-        opt = RunOptions(64, bc, T_bc, batch, 4, rows, -1.0, int(keep_records), int(graph), 0, 0,
-                         0)
+        opt = RunOptions(precision, bc, T_bc, batch, depth, rows, T_ref, int(keep_records),
+                         int(graph), 0, int(cruise_cols), 0)
         h = C.c_void_p()
         rc = lib().lpbf_ensemble_create(
             ks, n, phys, Tm, Lx, Ly, tamb.ctypes.data, plan.ctypes.data if plan.size else None,

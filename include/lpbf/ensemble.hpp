@@ -25,6 +25,17 @@
 // differ in their constants (P, V, r0, eta, dSub, kmult, dt, h), plan
 // length and initial field; the grid, physics, boundary condition and Tm
 // are shared.  No snapshots (the manifest loop writes none).
+//
+// Precision 32 (baseline physics) steps the runs still running as one
+// lpbf_f32_batch: each launch advances every run by up to `depth` steps,
+// ending at batch ends, at any run's last step (the batch is then rebuilt
+// without the runs that ended) and, with cruise_cols, at any run's cruise
+// step.  After a launch whose last step is a run's cruise step, one
+// lpbf_launch_hot_cols_f32 copies the hot columns of all runs.  Fields are
+// bitwise those of FP32 DeviceRuns (the FP32 field does not depend on the
+// launch depth or band height), and W_sub is taken where such a DeviceRun
+// ends a launch when the launch boundaries agree (equal batch, depth and
+// run lengths).  Each run's theta origin is its Tamb (or T_ref if >= 0).
 #pragma once
 
 #include <vector>
@@ -46,9 +57,14 @@ struct EnsembleOptions {
     int bc = LPBF_BC_MIRROR;
     double T_bc = 300.0;      // Dirichlet edge temperature
     int batch = 64;           // steps per host synchronisation
-    int rows = 0;             // band height per wavefront (0: lpbf_step_ensemble_rows)
+    int rows = 0;             // band height per wavefront (0: lpbf_step_ensemble_rows;
+                              // precision 32: the FP32 kernel's band height, 0 = built-in)
     bool keep_records = false;
-    bool graph = true;        // replay full batches as one captured hipGraph
+    bool graph = true;        // FP64: replay full batches as one captured hipGraph
+    int precision = 64;       // 64 (FP64 ensemble launch) or 32 (batched FP32 kernel)
+    int depth = 4;            // FP32: steps per launch, 1..4
+    double T_ref = -1.0;      // FP32 theta origin (< 0: each run's Tamb)
+    bool cruise_cols = false; // FP32: launches also end at cruise steps
 };
 
 class Ensemble {
@@ -83,6 +99,7 @@ private:
     // copied for batch steps [c_lo, c_hi]
     struct Pending {
         int s0, bn, slot, c_lo, c_hi;
+        std::vector<char> col_at;     // precision 32: batch step ends a launch
     };
     EnsembleMember const& at(int m) const { return runs_.at(static_cast<std::size_t>(m)); }
     Pending enqueue(int s0, int bn, int slot);
@@ -90,6 +107,12 @@ private:
     void launches(int bn);
     void capture();
     void release() noexcept;
+    void* dalloc(std::size_t bytes, const char* what);   // device memory freed in release()
+    // precision 32
+    void setup_f32();
+    Pending enqueue_f32(int s0, int bn, int slot);
+    void rebuild_f32(int next);
+    int latest_f32(int m) const;      // buffer (0/1) holding run m's latest field
 
     Domain dom_;
     int phys_;
@@ -112,6 +135,25 @@ private:
     void* gexec_ = nullptr;
     std::vector<RunMetrics> met_;
     std::vector<std::vector<StepRecord>> rec_;
+    // precision 32: theta fields (2 per run, pitch ld_), the batch of the
+    // runs still running (active_), each run's latest buffer at the last
+    // rebuild (fcur_) and the launches since (flaunch_), per-step records
+    // [run][max_steps], the hot-column jobs of the active runs, staging for
+    // FP64 uploads and downloads
+    int ld_ = 0;
+    std::vector<float*> fbuf_;
+    std::vector<double> tref_;
+    std::vector<LaserTick*> d_plan_;
+    std::vector<unsigned char*> d_cruise_;
+    std::vector<std::vector<unsigned char>> cruise_;
+    void* fbatch_ = nullptr;          // lpbf_f32_batch*
+    std::vector<int> active_;
+    std::vector<int> fcur_;
+    int flaunch_ = 0;
+    bool fresh_ = false;              // the batch has not launched yet
+    StepStats* d_stats_ = nullptr;
+    void* d_jobs_ = nullptr;          // lpbf_f32_col_job[n]
+    double* d_stage_ = nullptr;
 };
 
 }  // namespace lpbf

@@ -93,8 +93,33 @@ int  lpbf_f32_batch_capture(lpbf_f32_batch* b, int step0, int n_steps, hipStream
  * number of launches); an even number of eager launches restores it. */
 int  lpbf_f32_batch_launch_graph(lpbf_f32_batch* b, hipStream_t stream);
 
+/* Enqueues steps [step0, step0+n_steps) of every instance as exactly one
+ * launch per launch group (1 <= n_steps <= the depth of every class), with
+ * no splitting for graph replay: every field moves to its other buffer.
+ * For callers that end launches at steps of their own choosing (the FP32
+ * ensemble ends them at batch ends, run ends and cruise steps). */
+int  lpbf_f32_batch_enqueue_launch(lpbf_f32_batch* b, int step0, int n_steps,
+                                   hipStream_t stream);
+
 /* 0 if theta0, 1 if theta1 holds instance i's latest field; < 0 on error. */
 int  lpbf_f32_batch_current(const lpbf_f32_batch* b, int instance);
+
+/* Hot columns of many FP32 fields after one step (the FP32 counterpart of
+ * the ensemble's per-step column copy, lpbf_ensemble_run.cols): for each job
+ * whose cruise[step] is nonzero and step < n_steps, writes the ny values
+ * T_ref + theta[q][y*ld + x_hot], x_hot = stats[step].arg % nx, to
+ * col + slot*col_pitch.  One workgroup per job; asynchronous on `stream`. */
+typedef struct lpbf_f32_col_job {
+    const float*         theta[2];  /* theta[q] holds the field after `step` */
+    const StepStats*     stats;     /* merged per-step statistics, by step */
+    const unsigned char* cruise;    /* per-step copy flags */
+    double*              col;       /* destination of slot 0 */
+    double               T_ref;     /* theta origin [K] */
+    int                  ld, nx, ny, n_steps;
+} lpbf_f32_col_job;
+
+int  lpbf_launch_hot_cols_f32(const lpbf_f32_col_job* d_jobs, int n_jobs, int q, int step,
+                              int slot, long long col_pitch, hipStream_t stream);
 
 /* Workgroups and launches per enqueue of n_steps (for traffic accounting). */
 int  lpbf_f32_batch_launches(const lpbf_f32_batch* b, int n_steps, int* n_launches,
@@ -103,8 +128,10 @@ int  lpbf_f32_batch_launches(const lpbf_f32_batch* b, int n_steps, int* n_launch
 #ifdef __cplusplus
 }
 static_assert(sizeof(lpbf_f32_instance) == 288, "lpbf_f32_instance layout");
+static_assert(sizeof(lpbf_f32_col_job) == 64, "lpbf_f32_col_job layout");
 #else
 _Static_assert(sizeof(lpbf_f32_instance) == 288, "lpbf_f32_instance layout");
+_Static_assert(sizeof(lpbf_f32_col_job) == 64, "lpbf_f32_col_job layout");
 #endif
 
 #endif /* LPBF_BATCH_H */

@@ -111,8 +111,8 @@ void lpbf_launch_step_ex_plan(const double* Tin, double* Tout, const StepConst* 
  * j = 0..n-1 (every node carries its own j, so there is no device counter
  * and no atomic): the caller copies the batch's laser ticks and cruise flags
  * to d_ticks[0..n) and d_cruise[0..n) (device) before each replay.  Launch j
- * steps Tin -> Tout with the laser of d_ticks[j] and writes its block
- * records into block_stats[(j % 2) * nblk ...] (nblk =
+ * steps Tin -> Tout with the laser of d_ticks[j] and writes its records
+ * (one per wavefront) into block_stats[(j % 2) * nblk ...] (nblk =
  * lpbf_step_ex_batch_nblocks(nx, ny); block_stats holds 2 * nblk records);
  * one extra workgroup meanwhile finishes step j-1: it merges that step's
  * records into ring[j-1] and, when d_cruise[j-1] != 0, copies the column of

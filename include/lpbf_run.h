@@ -93,8 +93,14 @@ int  lpbf_run_records(const lpbf_run* run, int first, int n, lpbf_step_record* o
  * plan / x_csv / wrapped hold the runs' ticks back to back (run m's start
  * after the n_steps of runs 0..m-1), with the meaning of lpbf_run_create's;
  * Tamb[m] is run m's initial uniform field (NULL: 300 K).  opt: precision
- * must be 64; bc, T_bc, batch, rows (band height of the ensemble launch,
- * 0..1024, 0 = lpbf_step_ensemble_rows), keep_records and graph are used.
+ * 64 uses bc, T_bc, batch, rows (band height of the ensemble launch,
+ * 0..1024, 0 = lpbf_step_ensemble_rows), keep_records and graph.  Precision
+ * 32 (baseline physics) steps the runs still running as one lpbf_f32_batch
+ * (include/lpbf_batch.h) with up to `depth` steps per launch; launches end
+ * at batch ends, at any run's last step and, with cruise_cols, at any run's
+ * cruise step; W_sub is taken at launch ends; rows is the FP32 band height
+ * (0 = built-in), T_ref the theta origin (< 0: each run's Tamb); graph is
+ * ignored.  Its fields equal FP32 lpbf_run_* fields bit for bit.
  * Snapshots are not taken.  Calls on one handle must not overlap. */
 typedef struct lpbf_ensemble lpbf_ensemble;
 int  lpbf_ensemble_create(const StepConst* k, int n_runs, int phys, double Tm, double Lx,

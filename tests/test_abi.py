@@ -80,13 +80,16 @@ def test_ensemble_create_rejects_bad_runs_before_touching_a_device():
                      ([member()], dict(phys=3)),                 # unknown physics
                      ([member()], dict(bc=5)),                   # unknown boundary
                      ([member()], dict(batch=0)),
+                     ([member()], dict(precision=32, phys=1)),   # FP32: baseline only
+                     ([member()], dict(precision=32, depth=5)),
+                     ([member()], dict(precision=32, depth=4, rows=6)),   # rows < 2*depth
                      ([member(), member(nx=48)], {})):           # grids differ
         with pytest.raises((h.LpbfError, ValueError)):
             h.Ensemble(runs, Tm=1609.0, Lx=1e-3, Ly=1e-3, **kw)
     h.lib().lpbf_get_last_error()
-    # precision other than 64 is refused by the C entry point itself
+    # a precision other than 64 or 32 is refused by the C entry point itself
     lib, k = h.lib(), (h.StepConst * 1)(member()["k"])
-    opt = h.RunOptions(32, 0, 300.0, 64, 4, 0, -1.0, 0, 1, 0, 0, 0)
+    opt = h.RunOptions(16, 0, 300.0, 64, 4, 0, -1.0, 0, 1, 0, 0, 0)
     n = (C.c_int * 1)(4)
     plan = np.zeros((4, 3))
     out = C.c_void_p()
@@ -113,6 +116,9 @@ def test_batch_create_rejects_bad_instances_before_touching_a_device():
     assert lib.lpbf_f32_batch_create(arr, 0, 0, C.byref(out)) == 1000
     assert lib.lpbf_get_last_error() == 1000
     assert lib.lpbf_f32_batch_enqueue(None, 0, 1, None) == 1000
+    assert lib.lpbf_f32_batch_enqueue_launch(None, 0, 1, None) == 1000
+    assert lib.lpbf_launch_hot_cols_f32(None, 1, 0, 0, 0, 0, None) == 1000
+    assert lib.lpbf_launch_hot_cols_f32(1024, 1, 2, 0, 0, 0, None) == 1000     # q not 0/1
     lib.lpbf_get_last_error()
 
 

@@ -134,9 +134,10 @@ def test_lpbf_gpu_manifest_ensemble_equals_row_loop(tmp_path):
     assert [x[1] for x in rows_e] == ["a", "b", "c", "d"]
     # every column but wall_ms (index 14) identical
     assert [x[:14] + x[15:] for x in rows_e] == [x[:14] + x[15:] for x in rows]
+    # the FP32 path runs baseline physics only: the ext rows are refused
     bad = subprocess.run([EXE, "--manifest=%s" % man, "--ensemble", "--precision=32"],
                          capture_output=True, text=True, timeout=120)
-    assert bad.returncode == 1 and "--ensemble" in bad.stderr
+    assert bad.returncode == 2 and "baseline physics" in bad.stderr
 
 
 @pytest.mark.parametrize("rows", [1, 3, 120])
