@@ -123,47 +123,100 @@ __global__ void k_pack(const float* __restrict__ p, uint8_t* __restrict__ pack) 
 
 // ---- conv 32 -> 32 on MFMA --------------------------------------------------
 
-enum { kModeFwd = 0, kModeBwd = 1 };
+// kModeFwd: forward, activation a only (inference); kModeFwdZ: forward,
+// pre-activation z and a (training); kModeBwd: backward data.
+enum { kModeFwd = 0, kModeBwd = 1, kModeFwdZ = 2 };
 
 // X, out0, out1, zprev: [batch][128][128][32] bf16.  W: [9][32 out][32 in].
-// A workgroup owns two output rows of one window: it stages the four input
-// rows they touch (130 pixels with the zero border, 64 bytes each) in LDS
-// once, and each wavefront (one row half) reads its 16-byte operands from
-// there for all nine taps.
+// A workgroup owns a strip of `strip` output rows of one window and walks it
+// one row at a time; wavefront w takes pixels 32w .. 32w+31 of the row, all
+// 32 output channels (2 x 2 accumulator tiles; the weights of the nine taps
+// stay in registers).  The input rows live in a six-slot LDS ring (130
+// pixels with the zero border, 64 bytes each): three hold the rows the
+// current output row reads, three stream in from HBM for the rows ahead
+// (global_load_lds_dwordx4: lane l of a wavefront lands 16 bytes at the slot
+// offset + 16 l, so one instruction moves 16 pixels; nothing passes through
+// registers).  An input row is read once per strip (plus one halo row each
+// side), and its load has three rows of MFMAs to land in.  Each output tile
+// accumulates its nine taps in the same order whatever the strip height.
+constexpr int kConvCols = kW + 2;
+constexpr int kSlot = kConvCols * 4;            // uint4 per ring slot
+constexpr int kRing = 6;
+__device__ __forceinline__ int ring_slot(int yy) { return (yy + kRing) % kRing; }   // yy >= -1
+
+typedef __attribute__((address_space(3))) uint4 lds_uint4_t;
+
+// 16 bytes per lane from global memory straight into LDS: lane l writes at
+// LDS address lds_base + 16 l (lds_base wave-uniform, in M0).  Written as
+// asm so that the compiler, which would otherwise wait for the transfer
+// before every later LDS read (it cannot tell the ring slots apart), leaves
+// the wait to the explicit s_waitcnt before the barrier that publishes the
+// slot.  The instruction has no register result, and vector-memory
+// operations complete in issue order, so an operation the compiler does not
+// know about only makes its own vmcnt waits stricter.
+__device__ __forceinline__ void lds_dma16(const uint4* gptr, uint32_t lds_base) {
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off"
+                 :
+                 : "v"(gptr), "s"(lds_base)
+                 : "memory", "m0");
+}
+
+// Row yy of the window into its ring slot: wavefront w moves pixels
+// 32w .. 32w+31 (two instructions, kDmaOps); a row outside the window is
+// written as zeros (no memory operation).  Wave-uniform.
+constexpr int kDmaOps = 2;
+__device__ __forceinline__ void conv_row_to_lds(const uint4* __restrict__ Xv, uint4* s_x, int yy,
+                                                int wave, int lane) {
+    uint4* slot = s_x + ring_slot(yy) * kSlot + 4;   // pixel 0
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int px0 = wave * 32 + h * 16;
+        if (yy >= 0 && yy < kW) {
+            const uint32_t base = static_cast<uint32_t>(
+                reinterpret_cast<uintptr_t>((lds_uint4_t*)(slot + px0 * 4)));
+            lds_dma16(Xv + (yy * kW + px0) * 4 + lane, __builtin_amdgcn_readfirstlane(base));
+        } else {
+            slot[px0 * 4 + lane] = make_uint4(0, 0, 0, 0);
+        }
+    }
+}
+
+// Waits until at most n vector-memory operations of this wavefront are
+// outstanding (n an immediate).
+template <int N>
+__device__ __forceinline__ void vm_wait() {
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kT) void k_conv_mfma(const uint16_t* __restrict__ X,
                                                   const uint16_t* __restrict__ W,
                                                   const float* __restrict__ bias,
                                                   const uint16_t* __restrict__ zprev,
                                                   uint16_t* __restrict__ out0,
-                                                  uint16_t* __restrict__ out1, int batch) {
-    constexpr int kRowsIn = 4, kCols = kW + 2;
-    __shared__ __attribute__((aligned(16))) uint4 s_x[kRowsIn * kCols * 4];
+                                                  uint16_t* __restrict__ out1, int batch,
+                                                  int strip) {
+    // vector-memory operations a wavefront issues per output row after that
+    // row's transfer: the stores of one or two output tensors (four each),
+    // or the backward form's four z loads and four stores
+    constexpr int kRowOps = MODE == kModeFwd ? 4 : 8;
+    __shared__ __attribute__((aligned(16))) uint4 s_x[kRing * kSlot];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int img = blockIdx.x / (kW / 2), r0 = (blockIdx.x % (kW / 2)) * 2;
+    const int nstrip = kW / strip;
+    const int img = blockIdx.x / nstrip, ys = (blockIdx.x % nstrip) * strip;
     if (img >= batch) return;
     const int lr = lane & 15, lg = lane >> 4;
-
-    // all loads in flight before the first LDS store
+    const int ye = ys + strip;                       // one past the strip's last row
     const uint4* Xv = reinterpret_cast<const uint4*>(X) + static_cast<size_t>(img) * kPix * 4;
-    constexpr int kStage = kRowsIn * kCols * 4, kIt = (kStage + kT - 1) / kT;
-    uint4 v[kIt];
-#pragma unroll
-    for (int it = 0; it < kIt; ++it) {
-        const int q = threadIdx.x + it * kT;
-        const int rr = q / (kCols * 4), rem = q % (kCols * 4);
-        const int px = rem / 4 - 1, c = rem & 3;
-        const int yy = r0 - 1 + rr;
-        v[it] = make_uint4(0, 0, 0, 0);
-        if (q < kStage && yy >= 0 && yy < kW && px >= 0 && px < kW)
-            v[it] = Xv[(yy * kW + px) * 4 + c];
+
+    // the zero border pixels of every slot (never overwritten); the first
+    // row's three input rows and the two after them
+    if (threadIdx.x < 8 * kRing) {
+        const int sl = threadIdx.x >> 3, e = threadIdx.x & 7;
+        s_x[sl * kSlot + (e < 4 ? e : kSlot - 8 + e)] = make_uint4(0, 0, 0, 0);
     }
-#pragma unroll
-    for (int it = 0; it < kIt; ++it) {
-        const int q = threadIdx.x + it * kT;
-        if (q < kStage) s_x[q] = v[it];
-    }
+    for (int yy = ys - 1; yy <= min(ys + 3, ye); ++yy) conv_row_to_lds(Xv, s_x, yy, wave, lane);
 
     bf16x8 w[9][2];
 #pragma unroll
@@ -172,60 +225,99 @@ __global__ __launch_bounds__(kT) void k_conv_mfma(const uint16_t* __restrict__ X
         for (int ct = 0; ct < 2; ++ct)
             w[t][ct] = as_bf8(*reinterpret_cast<const uint4*>(
                 W + (static_cast<size_t>(t) * kC + ct * 16 + lr) * kC + 8 * lg));
-
-    f32x4 acc[2][4];
+    const int x0 = wave * 32;
+    float bv[2][4];   // this lane's output channels' bias (forward)
 #pragma unroll
     for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-        for (int pt = 0; pt < 4; ++pt) acc[ct][pt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int i = 0; i < 4; ++i) bv[ct][i] = MODE != kModeBwd ? bias[ct * 16 + 4 * lg + i] : 0.f;
+    vm_wait<0>();
     __syncthreads();
 
-    const int lrow = wave >> 1, row = r0 + lrow, x0 = (wave & 1) * 64;
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        const int dy = t / 3, dx = t % 3;   // LDS row lrow + dy, column px + dx
-        bf16x8 xv[4];
-#pragma unroll
-        for (int pt = 0; pt < 4; ++pt)
-            xv[pt] = as_bf8(s_x[((lrow + dy) * kCols + x0 + pt * 16 + lr + dx) * 4 + lg]);
+    for (int row = ys; row < ye; ++row) {
+        // the input row of output row + 3, into the slot row - 2 vacated
+        const bool dma = row + 3 < ye;               // workgroup-uniform
+        if (dma) conv_row_to_lds(Xv, s_x, row + 4, wave, lane);
+        // backward: the previous layer's z at this row's pixels, read before
+        // the MFMAs so that its latency hides behind them
+        uint2 zp[2][2];
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-            for (int pt = 0; pt < 4; ++pt)
-                acc[ct][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[t][ct], xv[pt], acc[ct][pt],
-                                                                      0, 0, 0);
-    }
+            for (int pt = 0; pt < 2; ++pt)
+                zp[ct][pt] = MODE != kModeBwd
+                                 ? make_uint2(0, 0)
+                                 : *reinterpret_cast<const uint2*>(
+                                       zprev + (static_cast<size_t>(img) * kPix + row * kW + x0 +
+                                                pt * 16 + lr) * kC + ct * 16 + 4 * lg);
 
-    // lane holds channels co0..co0+3 of pixel x0 + pt*16 + lr
+        f32x4 acc[2][2];
 #pragma unroll
-    for (int ct = 0; ct < 2; ++ct) {
-        const int co0 = ct * 16 + 4 * lg;
+        for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-        for (int pt = 0; pt < 4; ++pt) {
-            const size_t off = (static_cast<size_t>(img) * kPix + row * kW + x0 + pt * 16 + lr) * kC + co0;
-            if (MODE == kModeFwd) {
-                float z[4], a[4];
+            for (int pt = 0; pt < 2; ++pt) acc[ct][pt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    z[i] = acc[ct][pt][i] + bias[co0 + i];
-                    a[i] = gelu(z[i]);
-                }
-                if (out0)
+        for (int t = 0; t < 9; ++t) {
+            const int dy = t / 3, dx = t % 3;   // input row row - 1 + dy, column px + dx
+            const int sl = ring_slot(row - 1 + dy);
+            bf16x8 xv[2];
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt)
+                xv[pt] = as_bf8(s_x[sl * kSlot + (x0 + pt * 16 + lr + dx) * 4 + lg]);
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                for (int pt = 0; pt < 2; ++pt)
+                    acc[ct][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[t][ct], xv[pt],
+                                                                          acc[ct][pt], 0, 0, 0);
+        }
+
+        // lane holds channels co0..co0+3 of pixel x0 + pt*16 + lr
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            const int co0 = ct * 16 + 4 * lg;
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt) {
+                const size_t off =
+                    (static_cast<size_t>(img) * kPix + row * kW + x0 + pt * 16 + lr) * kC + co0;
+                if (MODE != kModeBwd) {
+                    float z[4], a[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        z[i] = acc[ct][pt][i] + bv[ct][i];
+                        a[i] = gelu(z[i]);
+                    }
+                    if (MODE == kModeFwdZ)
+                        *reinterpret_cast<uint2*>(out0 + off) =
+                            make_uint2(pack_bf2(z[0], z[1]), pack_bf2(z[2], z[3]));
+                    *reinterpret_cast<uint2*>(out1 + off) =
+                        make_uint2(pack_bf2(a[0], a[1]), pack_bf2(a[2], a[3]));
+                } else {
+                    const uint2 zq = zp[ct][pt];
+                    const uint16_t zs[4] = {uint16_t(zq.x & 0xffff), uint16_t(zq.x >> 16),
+                                            uint16_t(zq.y & 0xffff), uint16_t(zq.y >> 16)};
+                    float d[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) d[i] = acc[ct][pt][i] * gelu_grad(bf2f(zs[i]));
                     *reinterpret_cast<uint2*>(out0 + off) =
-                        make_uint2(pack_bf2(z[0], z[1]), pack_bf2(z[2], z[3]));
-                *reinterpret_cast<uint2*>(out1 + off) =
-                    make_uint2(pack_bf2(a[0], a[1]), pack_bf2(a[2], a[3]));
-            } else {
-                const uint2 zp = *reinterpret_cast<const uint2*>(zprev + off);
-                const uint16_t zs[4] = {uint16_t(zp.x & 0xffff), uint16_t(zp.x >> 16),
-                                        uint16_t(zp.y & 0xffff), uint16_t(zp.y >> 16)};
-                float d[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) d[i] = acc[ct][pt][i] * gelu_grad(bf2f(zs[i]));
-                *reinterpret_cast<uint2*>(out0 + off) =
-                    make_uint2(pack_bf2(d[0], d[1]), pack_bf2(d[2], d[3]));
+                        make_uint2(pack_bf2(d[0], d[1]), pack_bf2(d[2], d[3]));
+                }
             }
         }
+        // output row + 1 reads input row + 2, whose transfer was issued two
+        // rows ago (row - 2 + 4; the first two rows' come from the
+        // prologue, already waited for).  Operations issued after it: this
+        // row's and the previous row's transfers (if any) and the three
+        // rows' other operations; vector memory completes in issue order.
+        if (row + 1 < ye && row - 2 >= ys) {
+            const bool dma_prev = row + 2 < ye;      // row - 1 issued one
+            if (dma && dma_prev) vm_wait<3 * kRowOps + 2 * kDmaOps>();
+            else if (dma_prev) vm_wait<3 * kRowOps + kDmaOps>();
+            else vm_wait<3 * kRowOps>();
+        }
+        // after the barrier every wavefront's part of input row + 2 is in
+        // its slot, and every wavefront has finished with row - 1's slot
+        __syncthreads();
     }
 }
 
@@ -365,8 +457,103 @@ __global__ __launch_bounds__(kT) void k_dw_reduce(const float* __restrict__ part
 // ---- layer 6 (32 -> 1), VALU ------------------------------------------------
 
 // y = b6 + sum_t sum_ci a5[p + off(t)][ci] W6[t][ci]; with a target also the
-// loss gradient dy = 2 (y - target) / n_total and per-block sums of (y-t)^2.
+// loss gradient dy = 2 (y - target) / n_total and per-workgroup sums of
+// (y-t)^2.  A workgroup owns a strip of `strip` rows (even) of one window
+// and takes two rows at a time, one pixel per thread: the four a5 rows the
+// pair reads sit in LDS (staged once, 64 bytes per pixel, zero border), and
+// the two rows the next pair adds are loaded into registers while this pair
+// is summed.  Each pixel sums its 288 products in the order t, channel.
 __global__ __launch_bounds__(kT) void k_out_fwd(const uint16_t* __restrict__ a5,
+                                                const uint8_t* __restrict__ pack,
+                                                const float* __restrict__ target, float* __restrict__ y,
+                                                float* __restrict__ dy, double* __restrict__ lpart,
+                                                int batch, int strip) {
+    constexpr int kCols = kW + 2, kSlot = kCols * 4;
+    __shared__ __attribute__((aligned(16))) uint4 s_a[4 * kSlot];   // ring of four rows
+    const float* W6 = reinterpret_cast<const float*>(pack + kOffW6);
+    const float b6 = reinterpret_cast<const float*>(pack + kOffB6)[0];
+    const long long n = static_cast<long long>(batch) * kPix;
+    const int nstrip = kW / strip;
+    const int img = blockIdx.x / nstrip, ys = (blockIdx.x % nstrip) * strip;
+    const uint4* Av = reinterpret_cast<const uint4*>(a5) + static_cast<size_t>(img) * kPix * 4;
+    auto slot_of = [](int yy) { return (yy + 4) & 3; };
+    // rows y0 .. y0+nrow-1 into registers / into their slots
+    auto fetch = [&](uint4* v, int nit, int y0, int nrow) {
+        for (int it = 0; it < nit; ++it) {
+            const int q = threadIdx.x + it * kT;
+            const int rr = q / kSlot, rem = q % kSlot;
+            const int px = rem / 4 - 1, c = rem & 3, yy = y0 + rr;
+            v[it] = make_uint4(0, 0, 0, 0);
+            if (rr < nrow && yy >= 0 && yy < kW && px >= 0 && px < kW)
+                v[it] = Av[(yy * kW + px) * 4 + c];
+        }
+    };
+    auto stash = [&](const uint4* v, int nit, int y0, int nrow) {
+        for (int it = 0; it < nit; ++it) {
+            const int q = threadIdx.x + it * kT;
+            const int rr = q / kSlot, rem = q % kSlot;
+            if (rr < nrow) s_a[slot_of(y0 + rr) * kSlot + rem] = v[it];
+        }
+    };
+    double d2 = 0.0;
+    if (img < batch) {
+        {
+            constexpr int kIt = (4 * kSlot + kT - 1) / kT;
+            uint4 v[kIt];
+            fetch(v, kIt, ys - 1, 4);
+            stash(v, kIt, ys - 1, 4);
+        }
+        __syncthreads();
+        const int lrow = threadIdx.x / kW, px = threadIdx.x % kW;
+        constexpr int kNext = (2 * kSlot + kT - 1) / kT;
+        for (int r0 = ys; r0 < ys + strip; r0 += 2) {
+            const bool more = r0 + 2 < ys + strip;   // workgroup-uniform
+            uint4 nv[kNext];
+            if (more) fetch(nv, kNext, r0 + 3, 2);
+            const int py = r0 + lrow;
+            float acc = b6;
+            for (int t = 0; t < 9; ++t) {
+                // pixel px + dx - 1 of row py + dy - 1 is LDS column px + dx
+                const uint4* src =
+                    s_a + slot_of(py + t / 3 - 1) * kSlot + (px + t % 3) * 4;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const uint4 v = src[c];
+                    const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        acc = fmaf(bf2f(uint16_t(u[e] & 0xffff)), W6[t * kC + c * 8 + 2 * e], acc);
+                        acc = fmaf(bf2f(uint16_t(u[e] >> 16)), W6[t * kC + c * 8 + 2 * e + 1], acc);
+                    }
+                }
+            }
+            const long long p = static_cast<long long>(img) * kPix + py * kW + px;
+            y[p] = acc;
+            if (target) {
+                const float d = acc - target[p];
+                dy[p] = 2.f * d / static_cast<float>(n);
+                d2 += static_cast<double>(d) * d;
+            }
+            if (more) {
+                __syncthreads();   // the pair's reads of rows r0-1 and r0 are done
+                stash(nv, kNext, r0 + 3, 2);
+                __syncthreads();
+            }
+        }
+    }
+    if (target) {
+        __shared__ double s[kT / 64];
+        d2 = wave_sum(d2);
+        if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = d2;
+        __syncthreads();
+        if (threadIdx.x == 0) lpart[blockIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
+    }
+}
+
+// The same sums with one pixel per thread and its nine neighbours read from
+// global memory: for small batches, whose few strips would leave most CUs
+// idle (the two-level run's single window).  Per-block sums of (y-t)^2.
+__global__ __launch_bounds__(kT) void k_out_fwd_px(const uint16_t* __restrict__ a5,
                                                 const uint8_t* __restrict__ pack,
                                                 const float* __restrict__ target, float* __restrict__ y,
                                                 float* __restrict__ dy, double* __restrict__ lpart,
@@ -471,30 +658,37 @@ __global__ __launch_bounds__(kT) void k_out_bwd_data(const float* __restrict__ d
 }
 
 // Per workgroup over its rows: dW6[t][ci] = sum a5[p+off(t)][ci] dy[p] and
-// db6 = sum dy (slot 288).  partial[wg][289].  Thread (r, ci), r = tap row,
-// walks the staged a5 row once and feeds the three taps of that row from a
-// sliding window of dy (one LDS read of a and one of dy per column).
+// db6 = sum dy (slot 288).  partial[wg][289].  Thread (g, cp) owns channel
+// pair 2cp, 2cp+1 and pixels 8g .. 8g+7 of each row: per row it reads ten
+// pixels (two of halo) of each of the three staged a5 rows as channel pairs
+// and its eight dy values, and accumulates all nine taps (144 FMAs), so all
+// 256 threads work.  The next row's a5 rows and dy are loaded into
+// registers while the current row is summed.  The 16 pixel groups are added
+// in a fixed order at the end (deterministic).
 __global__ __launch_bounds__(kT) void k_out_dw(const uint16_t* __restrict__ a5,
                                                const float* __restrict__ dy,
                                                float* __restrict__ partial, int rows,
                                                int rows_per_wg) {
+    constexpr int kCols = kW + 2, kA = 3 * kCols * 4, kItA = (kA + kT - 1) / kT;
     __shared__ float s_dy[kW];
-    __shared__ __attribute__((aligned(16))) uint16_t s_a[3 * (kW + 2) * kPitch];
+    __shared__ __attribute__((aligned(16))) uint16_t s_a[3 * kCols * kPitch];
+    __shared__ float s_red[16][9][kC];
     const int tid = threadIdx.x;
+    const int cp = tid & 15, g = tid >> 4;           // channel pair, pixel group
     const int r0 = blockIdx.x * rows_per_wg, r1 = min(r0 + rows_per_wg, rows);
-    const int tr = tid / kC, ci = tid % kC;          // tap row 0..2 for tid < 96
-    float acc[3] = {0.f, 0.f, 0.f};
+    float acc[9][2];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[t][0] = acc[t][1] = 0.f;
     float accb = 0.f;
-    for (int gr = r0; gr < r1; ++gr) {
+
+    // a5 rows y-1 .. y+1 of row gr (zero outside the window) and its dy
+    auto fetch = [&](int gr, uint4* va, float& dyv) {
         const int img = gr / kW, y = gr % kW;
-        // all loads of the row in flight before the LDS stores
-        constexpr int kA = 3 * (kW + 2) * 4, kItA = (kA + kT - 1) / kT;
-        uint4 va[kItA];
-        const float dyv = tid < kW ? dy[static_cast<size_t>(img) * kPix + y * kW + tid] : 0.f;
+        dyv = tid < kW ? dy[static_cast<size_t>(img) * kPix + y * kW + tid] : 0.f;
 #pragma unroll
         for (int it = 0; it < kItA; ++it) {
             const int q = tid + it * kT;
-            const int rr = q / ((kW + 2) * 4), rem = q % ((kW + 2) * 4);
+            const int rr = q / (kCols * 4), rem = q % (kCols * 4);
             const int px = rem / 4 - 1, c = rem & 3;
             const int yy = y + rr - 1;
             va[it] = make_uint4(0, 0, 0, 0);
@@ -502,41 +696,73 @@ __global__ __launch_bounds__(kT) void k_out_dw(const uint16_t* __restrict__ a5,
                 va[it] = *reinterpret_cast<const uint4*>(
                     a5 + (static_cast<size_t>(img) * kPix + yy * kW + px) * kC + c * 8);
         }
-        __syncthreads();
+    };
+    uint4 va[kItA];
+    float dyv = 0.f;
+    if (r0 < r1) fetch(r0, va, dyv);
+    for (int gr = r0; gr < r1; ++gr) {
+        __syncthreads();   // the previous row's reads are done
         if (tid < kW) s_dy[tid] = dyv;
 #pragma unroll
         for (int it = 0; it < kItA; ++it) {
             const int q = tid + it * kT;
             if (q < kA) {
-                const int rr = q / ((kW + 2) * 4), rem = q % ((kW + 2) * 4);
-                const int px = rem / 4 - 1, c = rem & 3;
-                uint2* d = reinterpret_cast<uint2*>(s_a + (rr * (kW + 2) + px + 1) * kPitch + c * 8);
+                const int rr = q / (kCols * 4), rem = q % (kCols * 4);
+                const int col = rem / 4, c = rem & 3;
+                uint2* d = reinterpret_cast<uint2*>(s_a + (rr * kCols + col) * kPitch + c * 8);
                 d[0] = make_uint2(va[it].x, va[it].y);
                 d[1] = make_uint2(va[it].z, va[it].w);
             }
         }
         __syncthreads();
-        if (tid < 3 * kC) {
-            float d1 = 0.f, d2 = 0.f;                  // dy[c-1], dy[c-2]
-            for (int c = 0; c < kW + 2; ++c) {
-                const float d0 = c < kW ? s_dy[c] : 0.f;
-                const float a = bf2f(s_a[(tr * (kW + 2) + c) * kPitch + ci]);
-                acc[0] = fmaf(a, d0, acc[0]);           // dx = 0: pixel c
-                acc[1] = fmaf(a, d1, acc[1]);           // dx = 1: pixel c - 1
-                acc[2] = fmaf(a, d2, acc[2]);           // dx = 2: pixel c - 2
-                d2 = d1;
-                d1 = d0;
+        if (gr + 1 < r1) fetch(gr + 1, va, dyv);    // in flight while this row is summed
+        float d[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) d[j] = s_dy[8 * g + j];
+        if (cp == 0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) accb += d[j];
+        }
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr) {
+            // LDS column c holds pixel c - 1: pixel 8g + j + dx - 1 is column 8g + j + dx
+            float a0[10], a1[10];
+#pragma unroll
+            for (int j = 0; j < 10; ++j) {
+                const uint32_t v = *reinterpret_cast<const uint32_t*>(
+                    s_a + (rr * kCols + 8 * g + j) * kPitch + 2 * cp);
+                a0[j] = bf2f(uint16_t(v & 0xffff));
+                a1[j] = bf2f(uint16_t(v >> 16));
             }
-        } else if (tid == 3 * kC) {
-            for (int px = 0; px < kW; ++px) accb += s_dy[px];
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    acc[rr * 3 + dx][0] = fmaf(a0[j + dx], d[j], acc[rr * 3 + dx][0]);
+                    acc[rr * 3 + dx][1] = fmaf(a1[j + dx], d[j], acc[rr * 3 + dx][1]);
+                }
         }
     }
-    float* P = partial + static_cast<size_t>(blockIdx.x) * (9 * kC + 1);
-    if (tid < 3 * kC) {
+    // the 16 pixel groups in a fixed order
 #pragma unroll
-        for (int dx = 0; dx < 3; ++dx) P[(tr * 3 + dx) * kC + ci] = acc[dx];
-    } else if (tid == 3 * kC) {
-        P[9 * kC] = accb;
+    for (int t = 0; t < 9; ++t) {
+        s_red[g][t][2 * cp] = acc[t][0];
+        s_red[g][t][2 * cp + 1] = acc[t][1];
+    }
+    __shared__ float s_b[16];
+    if (cp == 0) s_b[g] = accb;
+    __syncthreads();
+    float* P = partial + static_cast<size_t>(blockIdx.x) * (9 * kC + 1);
+    for (int e = tid; e < 9 * kC; e += kT) {
+        const int t = e / kC, ci = e % kC;
+        float sum = 0.f;
+        for (int q = 0; q < 16; ++q) sum += s_red[q][t][ci];
+        P[e] = sum;
+    }
+    if (tid == 0) {
+        float sb = 0.f;
+        for (int q = 0; q < 16; ++q) sb += s_b[q];
+        P[9 * kC] = sb;
     }
 }
 
@@ -639,12 +865,46 @@ Work layout(void* base, int batch, bool training) {
 
 bool batch_ok(int b) { return b >= 1 && b <= 4096; }
 
+// Strip height of the conv launches: one row per workgroup while that
+// gives at most ~1k workgroups (small batches are latency-bound and want
+// every CU), taller strips beyond (fewer halo rows, loads overlapped with
+// more MFMAs); a power of two dividing 128.
+int conv_strip(int batch) {
+    int s = 1;
+    while (s < kW && static_cast<long long>(batch) * (kW / s) > 1024) s *= 2;
+    return s;
+}
+
+// The output layer: strips of at least two rows (like conv_strip) once the
+// batch fills the chip with them, one pixel per thread below.  Both sum each
+// pixel's products in the same order.  Returns the number of loss partials.
+int out_fwd(const uint16_t* a5, const uint8_t* pack, const float* target, float* y, float* dy,
+            double* lpart, int batch, hipStream_t s) {
+    const int strip = conv_strip(batch);
+    if (strip == 1) {
+        const int nb = static_cast<int>((static_cast<long long>(batch) * kPix + kT - 1) / kT);
+        hipLaunchKernelGGL(k_out_fwd_px, dim3(nb), dim3(kT), 0, s, a5, pack, target, y, dy, lpart,
+                           batch);
+        return nb;
+    }
+    const int nw = batch * (kW / strip);
+    hipLaunchKernelGGL(k_out_fwd, dim3(nw), dim3(kT), 0, s, a5, pack, target, y, dy, lpart, batch,
+                       strip);
+    return nw;
+}
+
 void conv_fwd(const uint16_t* X, const uint8_t* pack, int l, uint16_t* z, uint16_t* a, int batch,
               hipStream_t s) {
-    hipLaunchKernelGGL(k_conv_mfma<kModeFwd>, dim3(batch * (kW / 2)), dim3(kT), 0, s, X,
-                       reinterpret_cast<const uint16_t*>(pack + kOffWF) + static_cast<size_t>(l) * kWsz,
-                       reinterpret_cast<const float*>(pack + kOffB) + l * kC,
-                       static_cast<const uint16_t*>(nullptr), z, a, batch);
+    const int strip = conv_strip(batch);
+    const uint16_t* Wl =
+        reinterpret_cast<const uint16_t*>(pack + kOffWF) + static_cast<size_t>(l) * kWsz;
+    const float* bl = reinterpret_cast<const float*>(pack + kOffB) + l * kC;
+    if (z)
+        hipLaunchKernelGGL(k_conv_mfma<kModeFwdZ>, dim3(batch * (kW / strip)), dim3(kT), 0, s, X,
+                           Wl, bl, static_cast<const uint16_t*>(nullptr), z, a, batch, strip);
+    else
+        hipLaunchKernelGGL(k_conv_mfma<kModeFwd>, dim3(batch * (kW / strip)), dim3(kT), 0, s, X,
+                           Wl, bl, static_cast<const uint16_t*>(nullptr), z, a, batch, strip);
 }
 
 void forward_impl(const uint8_t* pack, const uint16_t* x0, int batch, Work const& w, hipStream_t s) {
@@ -684,10 +944,7 @@ void lpbf_sg_forward(const void* d_pack, const void* d_x0, int batch, void* d_wo
     const Work w = layout(d_work, batch, false);
     const uint8_t* pack = static_cast<const uint8_t*>(d_pack);
     forward_impl(pack, static_cast<const uint16_t*>(d_x0), batch, w, stream);
-    const long long n = static_cast<long long>(batch) * kPix;
-    hipLaunchKernelGGL(k_out_fwd, dim3(static_cast<int>((n + kT - 1) / kT)), dim3(kT), 0, stream,
-                       w.act[9], pack, static_cast<const float*>(nullptr), d_y,
-                       static_cast<float*>(nullptr), static_cast<double*>(nullptr), batch);
+    out_fwd(w.act[9], pack, nullptr, d_y, nullptr, nullptr, batch, stream);
     check_launch();
 }
 
@@ -705,9 +962,8 @@ void lpbf_sg_loss_grad(const void* d_pack, const void* d_x0, const float* d_targ
     const long long n = static_cast<long long>(batch) * kPix;
     const int nb = static_cast<int>((n + kT - 1) / kT);
     float* y = d_y ? d_y : w.ypred;
-    hipLaunchKernelGGL(k_out_fwd, dim3(nb), dim3(kT), 0, stream, w.act[9], pack, d_target, y,
-                       w.dy, w.lpart, batch);
-    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(kT), 0, stream, w.lpart, nb, n, d_loss);
+    const int nwo = out_fwd(w.act[9], pack, d_target, y, w.dy, w.lpart, batch, stream);
+    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(kT), 0, stream, w.lpart, nwo, n, d_loss);
     const int rows = batch * kW;
     const int nwg = dw_nwg(rows), per = (rows + nwg - 1) / nwg;
     // layer 6: weight gradient and dz5
@@ -725,12 +981,13 @@ void lpbf_sg_loss_grad(const void* d_pack, const void* d_x0, const float* d_targ
         hipLaunchKernelGGL(k_dw_reduce, dim3(kDwSlots * kC * kC / 32), dim3(kT), 0, stream,
                            w.part, nwg, l, d_grad);
         if (l > 0) {
-            hipLaunchKernelGGL(k_conv_mfma<kModeBwd>, dim3(batch * (kW / 2)), dim3(kT), 0, stream,
-                               w.dz[cur],
+            const int strip = conv_strip(batch);
+            hipLaunchKernelGGL(k_conv_mfma<kModeBwd>, dim3(batch * (kW / strip)), dim3(kT), 0,
+                               stream, w.dz[cur],
                                reinterpret_cast<const uint16_t*>(pack + kOffWB) +
                                    static_cast<size_t>(l) * kWsz,
                                static_cast<const float*>(nullptr), w.act[l - 1], w.dz[cur ^ 1],
-                               static_cast<uint16_t*>(nullptr), batch);
+                               static_cast<uint16_t*>(nullptr), batch, strip);
             cur ^= 1;
         }
     }
