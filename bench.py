@@ -553,6 +553,59 @@ def surrogate_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def sg_kernels_bench(args):
+    """The surrogate's kernels at batch 256 of 128 x 128 windows (32 channels,
+    BF16 activations, FP32 accumulation), on synthetic windows and the initial
+    parameters: the forward pass (five MFMA conv layers and the output layer)
+    and the training pass (forward with z stored, loss, backward data and
+    weight gradients), each timed over `--steps` repetitions.  value = forward
+    windows per second; the roofline is the forward pass's algorithmic bytes
+    (every layer reads its input once and writes its output once) over its
+    time."""
+    import torch
+
+    from hedgehog_lpbf_amd import surrogate as sg
+    B = 256
+    reps = args.steps if args.steps is not None else 20
+    torch.manual_seed(0)
+    model = sg.Surrogate(seed=0)
+    x0 = (torch.rand((B, sg.WIN, sg.WIN, sg.CH), device="cuda") - 0.5).to(torch.bfloat16)
+    tg = torch.rand((B, sg.WIN, sg.WIN), device="cuda")
+
+    def timed(f):
+        for _ in range(args.warmup if args.warmup is not None else 2):
+            f()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    fwd_ms = timed(lambda: model.forward(x0))
+    train_ms = timed(lambda: model.loss_grad(x0, tg))
+    act = sg.WIN * sg.WIN * sg.CH * 2
+    fwd_bytes = B * (5 * 2 * act + act + sg.WIN * sg.WIN * 4)
+    mfma_flop = 5 * B * sg.WIN * sg.WIN * sg.CH * 9 * sg.CH * 2
+    print(json.dumps({
+        "metric": "surrogate windows/s (forward, batch 256)", "value": B / (fwd_ms * 1e-3),
+        "unit": "windows/s", "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+        "ms_per_step": fwd_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (fp32 accumulate)", "data": "synthetic",
+        "config": {"workload": "surrogate network, batch 256 of 128x128 windows, 32 channels, "
+                               "6 conv3x3 layers, initial parameters, random windows",
+                   "forward_ms": fwd_ms, "training_pass_ms": train_ms},
+        "roofline": {"bound": "hbm", "kernel": "k_conv_mfma x5 + k_out_fwd (batch 256 forward)",
+                     "achieved": fwd_bytes / (fwd_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": fwd_bytes / (fwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "alg_bytes": fwd_bytes,
+                     "mfma_TFLOPs": mfma_flop / (fwd_ms * 1e-3) / 1e12,
+                     "mfma_frac": mfma_flop / (fwd_ms * 1e-3) / 1e12 / 2500.0,
+                     "counter_summary": "profiles/k_conv_mfma_surrogate.md"},
+        "cpu_baseline": None}), flush=True)
+
+
 def copy_reference(args):
     """Plain device-to-device copy of an n x n FP32 plate, `steps` times."""
     import torch
@@ -761,7 +814,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="plate8k_f32",
                     choices=["plate8k_f32", "plate8k_f64", "batch256", "case0", "ensemble",
-                             "ensemble_f32", "run8k_f64", "run8k_f32", "surrogate", "copy"],
+                             "ensemble_f32", "run8k_f64", "run8k_f32", "surrogate",
+                             "surrogate_kernels", "copy"],
                     help="plate8k_f32 (default): the FP32 plate, with the FP64 parity path "
                          "on the same plate in its `fp64` object; plate8k_f64: the FP64 "
                          "line alone; batch256: 256 uneven instances in one hipGraph launch; "
@@ -769,6 +823,8 @@ def main():
                          "ensemble: --runs manifest rows of that plate stepped together "
                          "(ensemble_f32: on the batched FP32 kernel); "
                          "run8k_f64 / run8k_f32: the device-resident run on the 8192^2 plate; "
+                         "surrogate_kernels: the network's forward and training passes at "
+                         "batch 256; "
                          "copy: a plain device copy of one n x n FP32 plate per step, the "
                          "reference for calibrating HBM byte counters")
     ap.add_argument("--depth", type=int, default=0,
@@ -811,6 +867,8 @@ def main():
         return run8k_bench(args, 64 if args.workload == "run8k_f64" else 32)
     if args.workload in ("ensemble", "ensemble_f32"):
         return ensemble_bench(args, 32 if args.workload == "ensemble_f32" else 64)
+    if args.workload == "surrogate_kernels":
+        return sg_kernels_bench(args)
     if args.workload == "surrogate":
         return surrogate_bench(args)
     if args.steps is None:

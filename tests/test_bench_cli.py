@@ -22,6 +22,6 @@ def test_more_than_one_gpu_is_refused_with_a_message():
 def test_every_workload_is_listed():
     r = _bench("--help")
     assert r.returncode == 0
-    for w in ("plate8k_f32", "plate8k_f64", "batch256", "case0", "ensemble", "run8k_f64",
-              "run8k_f32", "surrogate", "copy"):
+    for w in ("plate8k_f32", "plate8k_f64", "batch256", "case0", "ensemble", "ensemble_f32",
+              "run8k_f64", "run8k_f32", "surrogate", "surrogate_kernels", "copy"):
         assert w in r.stdout, w
