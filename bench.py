@@ -553,6 +553,90 @@ def surrogate_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def plate_ext_bench(args):
+    """The extended physics (temperature-dependent IN718 properties, latent
+    heat) on the `--n` x `--n` plate, dx 5 um, dt at the extended-physics
+    stability limit, the beam parked at the plate centre with the plate's
+    P and r0: the FP32 kernel (lpbf_launch_step_f32_ext, one step per launch,
+    fused statistics) and the FP64 parity kernel (lpbf_launch_step_ex,
+    phys = 1, block statistics), each over `--steps` steps after the
+    warm-up.  value = FP32 cell-updates per second."""
+    import numpy as np
+    import torch
+
+    import hedgehog_lpbf_amd as h
+    lib = h.lib()
+    mat, pr, dom, _, _, _ = plate_setup(args.n, args.n)
+    nx, ny = dom.nx, dom.ny
+    dt = h.dt_cfl_ext(mat, dom, 1.0)
+    k = h.make_const(mat, pr, dom, dt)
+    k.x_l, k.y_l, k.q_scale = 0.5 * dom.Lx, 0.5 * dom.Ly, 1.0
+    steps = args.steps if args.steps is not None else 200
+    warm = args.warmup if args.warmup is not None else 20
+    plan = np.tile(np.array([[k.x_l, k.y_l, 1.0]]), (warm + steps, 1))
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream(dev).cuda_stream
+    pdev = torch.from_numpy(plan).to(dev)
+    ld = (nx + 3) // 4 * 4
+    a = torch.empty((ny, ld), dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    lib.lpbf_launch_noise_f32(a.data_ptr(), nx, ny, ld, 0, 0.0, 0.5, st)
+    work = torch.empty(lib.lpbf_f32_ext_workspace_bytes(nx, ny) // 8 + 8, dtype=torch.float64,
+                       device=dev)
+    stats = torch.empty((warm + steps) * 7, dtype=torch.float64, device=dev)
+    bufs = [a, b]
+
+    def f32(s0, n):
+        for s in range(s0, s0 + n):
+            lib.lpbf_launch_step_f32_ext(bufs[0].data_ptr(), bufs[1].data_ptr(), ld, k, 300.0, 0,
+                                         300.0, mat.Tm, pdev.data_ptr(), s,
+                                         stats.data_ptr() + 56 * s, work.data_ptr(), st)
+            bufs.reverse()
+
+    def timed(f, s0, n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize(dev)
+        e0.record()
+        f(s0, n)
+        e1.record()
+        torch.cuda.synchronize(dev)
+        h.check()
+        return e0.elapsed_time(e1)
+
+    f32(0, warm)
+    ms32 = timed(f32, warm, steps)
+    A = torch.empty((ny, nx), dtype=torch.float64, device=dev)
+    B = torch.empty_like(A)
+    lib.lpbf_launch_noise_f64(A.data_ptr(), nx * ny, 0, 300.0, 0.5, st)
+    blk = torch.empty(lib.lpbf_step_ex_nblocks(nx, ny) * 7 + 7, dtype=torch.float64, device=dev)
+    fb = [A, B]
+
+    def f64(s0, n):
+        for _ in range(n):
+            lib.lpbf_launch_step_ex(fb[0].data_ptr(), fb[1].data_ptr(), k, 1, 0, 300.0, mat.Tm,
+                                    blk.data_ptr(), st)
+            fb.reverse()
+
+    f64(0, max(2, warm // 4))
+    n64 = max(10, steps // 4)
+    ms64 = timed(f64, 0, n64)
+    r32 = nx * ny * steps / (ms32 * 1e-3)
+    r64 = nx * ny * n64 / (ms64 * 1e-3)
+    print(json.dumps({
+        "metric": "cell-updates/s", "value": r32, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": warm, "ms_per_step": ms32 / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{nx}x{ny} plate, dx 5 um, extended physics (IN718 tables, "
+                               "latent heat, kmult 1), dt at its stability limit, beam parked "
+                               "at the centre, one step per launch with fused statistics",
+                   "dt_s": dt, "fp64_cell_updates_per_s": r64, "fp64_ms_per_step": ms64 / n64,
+                   "speedup_vs_fp64": r32 / r64},
+        "roofline": {"bound": "valu", "kernel": "k_step_f32_ext",
+                     "alg_bytes_per_step": 8 * nx * ny,
+                     "alg_frac_hbm": 8 * nx * ny / (ms32 / steps * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "cpu_baseline": None}), flush=True)
+
+
 def sg_kernels_bench(args):
     """The surrogate's kernels at batch 256 of 128 x 128 windows (32 channels,
     BF16 activations, FP32 accumulation), on synthetic windows and the initial
@@ -815,7 +899,7 @@ def main():
     ap.add_argument("--workload", default="plate8k_f32",
                     choices=["plate8k_f32", "plate8k_f64", "batch256", "case0", "ensemble",
                              "ensemble_f32", "run8k_f64", "run8k_f32", "surrogate",
-                             "surrogate_kernels", "copy"],
+                             "surrogate_kernels", "plate8k_f32_ext", "copy"],
                     help="plate8k_f32 (default): the FP32 plate, with the FP64 parity path "
                          "on the same plate in its `fp64` object; plate8k_f64: the FP64 "
                          "line alone; batch256: 256 uneven instances in one hipGraph launch; "
@@ -824,7 +908,8 @@ def main():
                          "(ensemble_f32: on the batched FP32 kernel); "
                          "run8k_f64 / run8k_f32: the device-resident run on the 8192^2 plate; "
                          "surrogate_kernels: the network's forward and training passes at "
-                         "batch 256; "
+                         "batch 256; plate8k_f32_ext: the extended physics on the plate, FP32 "
+                         "beside FP64; "
                          "copy: a plain device copy of one n x n FP32 plate per step, the "
                          "reference for calibrating HBM byte counters")
     ap.add_argument("--depth", type=int, default=0,
@@ -869,6 +954,8 @@ def main():
         return ensemble_bench(args, 32 if args.workload == "ensemble_f32" else 64)
     if args.workload == "surrogate_kernels":
         return sg_kernels_bench(args)
+    if args.workload == "plate8k_f32_ext":
+        return plate_ext_bench(args)
     if args.workload == "surrogate":
         return surrogate_bench(args)
     if args.steps is None:

@@ -183,6 +183,8 @@ SIGNATURES = {
     "lpbf_f32_batch_capture": (_i, [_vp, _i, _i, _vp]),
     "lpbf_f32_batch_launch_graph": (_i, [_vp, _vp]),
     "lpbf_f32_batch_current": (_i, [_vp, _i]),
+    "lpbf_f32_ext_workspace_bytes": (C.c_size_t, [_i, _i]),
+    "lpbf_launch_step_f32_ext": (_v, [_vp, _vp, _i, _KP, _d, _i, _d, _d, _vp, _i, _vp, _vp, _vp]),
     "lpbf_f32_batch_enqueue_launch": (_i, [_vp, _i, _i, _vp]),
     "lpbf_launch_hot_cols_f32": (_i, [_vp, _i, _i, _i, _i, C.c_longlong, _vp]),
     "lpbf_f32_batch_launches": (_i, [_vp, _i, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),

@@ -52,11 +52,11 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
         throw std::invalid_argument("DeviceRun: unknown boundary condition");
     if (phys_ != LPBF_PHYS_BASELINE && phys_ != LPBF_PHYS_EXTENDED)
         throw std::invalid_argument("DeviceRun: unknown physics");
-    if (opt_.precision == 32 && phys_ != LPBF_PHYS_BASELINE)
-        throw std::invalid_argument("DeviceRun: the FP32 path runs baseline physics only");
     if (opt_.batch < 1) opt_.batch = 1;
     if (opt_.precision == 32 && (opt_.depth < 1 || opt_.depth > 4))
         throw std::invalid_argument("DeviceRun: depth must be 1..4");
+    // the FP32 extended physics steps one step per launch
+    if (opt_.precision == 32 && phys_ == LPBF_PHYS_EXTENDED) opt_.depth = 1;
     if (opt_.rows < 0) throw std::invalid_argument("DeviceRun: rows must be >= 0");
     if (opt_.f64_depth == 0) opt_.f64_depth = 1;
     if (opt_.precision == 64 && (opt_.f64_depth < 1 || opt_.f64_depth > 4))
@@ -124,7 +124,9 @@ DeviceRun::DeviceRun(Domain dom, StepConst k, int phys, double Tm, std::vector<T
             const std::size_t F = static_cast<std::size_t>(ld_) * ny_ * sizeof(float);
             hip_ok(hipMalloc(&d_a_, F), "field");
             hip_ok(hipMalloc(&d_b_, F), "field");
-            const std::size_t W = lpbf_f32_workspace_bytes(nx_, ny_, opt_.depth, opt_.rows);
+            const std::size_t W = phys_ == LPBF_PHYS_EXTENDED
+                                      ? lpbf_f32_ext_workspace_bytes(nx_, ny_)
+                                      : lpbf_f32_workspace_bytes(nx_, ny_, opt_.depth, opt_.rows);
             if (W == 0) throw std::invalid_argument("DeviceRun: grid outside the FP32 range");
             hip_ok(hipMalloc(&d_blk_, W), "workspace");
             std::vector<LaserTick> lt = to_laser(plan_);
@@ -436,9 +438,14 @@ DeviceRun::Pending DeviceRun::enqueue(int s0, int bn, int slot) {
                 if (t.dump || (opt_.cruise_cols && cruise(t))) break;
                 ++L;
             }
-            lpbf_launch_step_f32(static_cast<const float*>(d_a_), static_cast<float*>(d_b_), ld_,
-                                 &k_, T_ref_, opt_.bc, opt_.T_bc, Tm_, d_plan_, s0 + i, L,
-                                 opt_.rows, d_ring_ + i, d_blk_, s);
+            if (phys_ == LPBF_PHYS_EXTENDED)
+                lpbf_launch_step_f32_ext(static_cast<const float*>(d_a_), static_cast<float*>(d_b_),
+                                         ld_, &k_, T_ref_, opt_.bc, opt_.T_bc, Tm_, d_plan_, s0 + i,
+                                         d_ring_ + i, d_blk_, s);
+            else
+                lpbf_launch_step_f32(static_cast<const float*>(d_a_), static_cast<float*>(d_b_),
+                                     ld_, &k_, T_ref_, opt_.bc, opt_.T_bc, Tm_, d_plan_, s0 + i, L,
+                                     opt_.rows, d_ring_ + i, d_blk_, s);
             std::swap(d_a_, d_b_);
             const int j = i + L - 1;
             const Tick& tk = plan_[static_cast<std::size_t>(s0 + j)];

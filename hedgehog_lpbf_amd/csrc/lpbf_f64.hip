@@ -74,66 +74,6 @@ struct Region {
 // One run of an ensemble (include/lpbf_kernels.h, lpbf_ensemble_run).
 using EnsMember = lpbf_ensemble_run;
 
-// Wavefront reductions on DPP lane moves and lane reads instead of LDS
-// permutes: within each 16-lane row, partners by quad_perm [1,0,3,2], then
-// [2,3,0,1], then the half-row and row mirrors (both lanes of a pair compute
-// a+b from the same two values, so every lane of a row ends with the same
-// bits); then the four row results, read from lanes 0, 16, 32, 48, in a
-// fixed order.  The result is the same in every lane, and the order does not
-// depend on the launch, so the FP64 sums are reproducible.
-template <int kCtrl>
-__device__ __forceinline__ int dpp_i32(int v) {
-    return __builtin_amdgcn_mov_dpp(v, kCtrl, 0xF, 0xF, false);
-}
-template <int kCtrl>
-__device__ __forceinline__ long long dpp_i64(long long v) {
-    const int lo = dpp_i32<kCtrl>(static_cast<int>(v));
-    const int hi = dpp_i32<kCtrl>(static_cast<int>(v >> 32));
-    return (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo);
-}
-template <int kCtrl>
-__device__ __forceinline__ double dpp_f64(double v) {
-    return __longlong_as_double(dpp_i64<kCtrl>(__double_as_longlong(v)));
-}
-__device__ __forceinline__ long long lane_i64(long long v, int l) {
-    const int lo = __builtin_amdgcn_readlane(static_cast<int>(v), l);
-    const int hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), l);
-    return (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo);
-}
-__device__ __forceinline__ double lane_f64(double v, int l) {
-    return __longlong_as_double(lane_i64(__double_as_longlong(v), l));
-}
-
-__device__ __forceinline__ double wave_total(double v) {
-    v = v + dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
-    v = v + dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
-    v = v + dpp_f64<0x141>(v);   // row_half_mirror
-    v = v + dpp_f64<0x140>(v);   // row_mirror
-    return (lane_f64(v, 0) + lane_f64(v, 16)) + (lane_f64(v, 32) + lane_f64(v, 48));
-}
-
-// Largest value with the smallest index on ties (exact, so any pairing
-// gives the serial result).
-__device__ __forceinline__ void take_hot(double& t, long long& a, double t2, long long a2) {
-    if (t2 > t || (t2 == t && a2 < a)) { t = t2; a = a2; }
-}
-template <int kCtrl>
-__device__ __forceinline__ void hot_step(double& t, long long& a) {
-    take_hot(t, a, dpp_f64<kCtrl>(t), dpp_i64<kCtrl>(a));
-}
-__device__ __forceinline__ void wave_hot(double& t, long long& a) {
-    hot_step<0xB1>(t, a);
-    hot_step<0x4E>(t, a);
-    hot_step<0x141>(t, a);
-    hot_step<0x140>(t, a);
-    double rt = lane_f64(t, 0);
-    long long ra = lane_i64(a, 0);
-#pragma unroll
-    for (int r = 16; r < kWave; r += 16) take_hot(rt, ra, lane_f64(t, r), lane_i64(a, r));
-    t = rt;
-    a = ra;
-}
-
 // Finishes step j of a batch: merges its nblk block records (in a fixed
 // order, so the sums are reproducible) into ring[j] and, on cruise steps,
 // copies the column of the step's maximum from `field` (the step's output)

@@ -23,7 +23,8 @@
 // instead, up to f64_depth steps per launch, a launch ending at every cruise
 // or dump step (the hot column and the snapshot need that step's field); the
 // fields are bitwise those of single steps.  Precision 32 runs
-// the temporally blocked FP32 kernel with up to `depth` steps per launch; a launch never
+// the temporally blocked FP32 kernel with up to `depth` steps per launch
+// (the extended physics: lpbf_launch_step_f32_ext, one step per launch); a launch never
 // crosses a dump step or a batch end, and the hot column (hence W_sub) is
 // taken at the last step of each launch only, or, with cruise_cols, at every
 // cruise step (launches also end there).

@@ -238,6 +238,23 @@ void   lpbf_launch_step_f32(const float* theta_in, float* theta_out, int ld,
                             int step0, int depth, int rows, StepStats* d_stats,
                             void* d_work, hipStream_t stream);
 
+/* FP32 form of the extended physics (the reference's phys = 1 update,
+ * include/step_task.hpp:132-187 and src/gpu/kernels.cu:125-147:
+ * temperature-dependent IN718 conductivity and heat capacity, latent heat
+ * over the mushy interval, k->kmult melt-pool conductivity enhancement) on
+ * the pitched FP32 theta field of lpbf_launch_step_f32: one step
+ * theta_in -> theta_out with the laser of d_plan[step], and if d_stats !=
+ * NULL the step's merged StepStats (sum_dT: the summed temperature change;
+ * sum_src, sum_loss: dt*q_in/(rho cp_eff h) and dt*(q_top+q_bot)/(rho
+ * cp_eff h), in K).  bc as for lpbf_launch_step_f32.  d_work must hold
+ * lpbf_f32_ext_workspace_bytes(nx, ny) bytes.  Its error against the FP64
+ * path is stated where it is tested (tests/test_gpu_f32_ext.py). */
+size_t lpbf_f32_ext_workspace_bytes(int nx, int ny);
+void   lpbf_launch_step_f32_ext(const float* theta_in, float* theta_out, int ld,
+                                const StepConst* k, double T_ref, int bc, double T_bc,
+                                double Tm, const LaserTick* d_plan, int step,
+                                StepStats* d_stats, void* d_work, hipStream_t stream);
+
 /* lpbf_launch_step_f32 without a caller workspace, for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  No counterpart in the reference.  The

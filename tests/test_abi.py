@@ -64,8 +64,7 @@ def test_run_create_rejects_bad_options_before_touching_a_device():
     k = h.StepConst()
     k.nx, k.ny, k.dt = 64, 32, 1e-6
     plan = np.zeros((4, 3))
-    for kw in (dict(precision=16), dict(bc=5), dict(precision=32, depth=9),
-               dict(precision=32, phys=1), dict(rows=-1)):
+    for kw in (dict(precision=16), dict(bc=5), dict(precision=32, depth=9), dict(rows=-1)):
         with pytest.raises(h.LpbfError):
             h.DeviceRun(k, Tm=1609.0, Lx=1e-3, Ly=1e-3, Tamb=300.0, plan=plan, **kw)
     assert h.lib().lpbf_get_last_error() == 0

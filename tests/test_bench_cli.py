@@ -23,5 +23,6 @@ def test_every_workload_is_listed():
     r = _bench("--help")
     assert r.returncode == 0
     for w in ("plate8k_f32", "plate8k_f64", "batch256", "case0", "ensemble", "ensemble_f32",
-              "run8k_f64", "run8k_f32", "surrogate", "surrogate_kernels", "copy"):
+              "run8k_f64", "run8k_f32", "surrogate", "surrogate_kernels", "plate8k_f32_ext",
+              "copy"):
         assert w in r.stdout, w
