@@ -283,7 +283,7 @@ def run8k_bench(args, precision):
     import hedgehog_lpbf_amd as h
     mat, pr, dom, dt, k, plan = plate_setup(args.n, args.n)
     depth = args.depth if args.depth > 0 else (2 if precision == 64 else 4)
-    rows = args.rows if args.rows > 0 or precision == 64 else 46
+    rows = args.rows if args.rows > 0 or precision == 64 else 92
     warm = args.warmup if args.warmup is not None else 64
     steps = args.steps if args.steps is not None else 1024
     kw = dict(f64_depth=depth) if precision == 64 else dict(precision=32, depth=depth)

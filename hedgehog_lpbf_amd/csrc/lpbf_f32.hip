@@ -9,8 +9,10 @@
 namespace lpbf_amd {
 namespace {
 
+// Occupancy targets: depth 4 with radiation fits three waves per SIMD (at
+// most 168 VGPRs), without it two; depths 1-3 fit four (at most 128).
 template <int K, int BC, bool RAD, bool RAGGED>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == 4 && !RAD ? 2 : 3))) void k_step_f32(F32Args a,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == 4 ? (RAD ? 3 : 2) : 4))) void k_step_f32(F32Args a,
                                                        const LaserTick* __restrict__ plan,
                                                        int step0) {
     step_tile<K, BC, RAD, false, RAGGED>(a, nullptr, nullptr, plan, step0,

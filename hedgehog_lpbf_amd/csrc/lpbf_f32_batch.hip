@@ -32,8 +32,9 @@ struct InstArgs {
     int pad_;
 };
 
+// depths 1-3 fit four waves per SIMD (at most 128 VGPRs)
 template <int K, int BC, bool RAD>
-__global__ __launch_bounds__(kThreads) void k_step_f32_multi(const InstArgs* __restrict__ inst,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == 4 ? 2 : 4))) void k_step_f32_multi(const InstArgs* __restrict__ inst,
                                                              const int* __restrict__ wg_inst,
                                                              int total_wg, int parity,
                                                              int step0) {

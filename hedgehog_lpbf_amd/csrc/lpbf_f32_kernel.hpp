@@ -14,12 +14,15 @@
 //     pipeline: loading row i advances level j to row i-j, so one pass over
 //     HBM performs K time steps (K <= 4 keeps every written column valid,
 //     since the invalid overlap grows by one column per level);
-//   * x neighbours are cross-lane shuffles, y neighbours the register window;
-//     edge conditions are fixed up per row, only in edge strips;
+//   * x neighbours are whole-wavefront DPP shifts folded into the adds, y
+//     neighbours the register window; mirror edges on widths that are a
+//     multiple of 4 use a reflected halo (no per-level edge work), other
+//     edge conditions are fixed up per level, only in edge strips;
 //   * per level, every owned cell-update feeds the step's statistics (peak T
 //     with its index, molten rows and count, energy sums): peak and sums per
-//     lane (FP32, the theta sum FP64), molten rows and count per wavefront
-//     from ballots, FP64 across lanes and workgroups;
+//     lane (FP32, the theta sum FP64; the peak's row staged in LDS), molten
+//     rows and count per wavefront from ballots, FP64 across lanes and
+//     workgroups;
 //   * four strips form a 256-thread workgroup; workgroup ids are remapped so
 //     consecutive tiles land on the same XCD (shared halo cache lines in L2).
 //
@@ -76,9 +79,9 @@ struct F32Args {
 // v_max x,x the compiler inserts before fmaxf of values it cannot prove
 // canonical (no operand here can be a NaN).
 __device__ __forceinline__ float max4(float4 v) {
-    float m, r;
-    asm("v_max3_f32 %0, %1, %2, %3" : "=v"(m) : "v"(v.x), "v"(v.y), "v"(v.z));
-    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(m), "v"(v.w));
+    float r;
+    asm("v_max3_f32 %0, %1, %2, %3\n\tv_max_f32 %0, %0, %4"
+        : "=&v"(r) : "v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w));
     return r;
 }
 
@@ -132,7 +135,8 @@ __device__ __forceinline__ void st_row4(float* p, float4 v) {
 
 struct LevelAcc {
     float bv;        // best theta of this lane
-    int bi;          // its row-major index (smallest on ties)
+    int bi;          // the body row it was found in; after the row loop its
+                     // row-major index (smallest on ties)
     int ymin, ymax;  // molten rows of the wavefront's owned cells (wave-uniform)
     int molten;      // molten owned cells of the wavefront (wave-uniform)
     double S;        // sum of theta over the owned cells after this step
@@ -161,13 +165,27 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
     const int task = wg * kWaves + wave;
     const int nx = a.nx, ny = a.ny;
     constexpr float kNegInf = -3.0e38f;
+    // Mirror edges on a width that is a multiple of 4 use a reflected halo:
+    // the ghost columns -4..-1 and nx..nx+3 are loaded as the mirror images
+    // of columns 3..0 and nx-1..nx-4 (one lane each, its four values
+    // reversed), and the ghost cells are updated like any other (the source
+    // at the mirrored column).  The update is symmetric, so every level's
+    // ghost column -1 equals column 0 bit for bit (and nx equals nx-1): the
+    // mirror condition holds without per-level fix-ups.  Each level spoils
+    // one more ghost column from the far side, so four stay exact for K <= 4.
+    constexpr bool kReflect = (BC == LPBF_BC_MIRROR) && !RAGGED;
 
     // per level: laser position in cell units, source scale, and the rows
     // [srlo, srhi] whose source can matter (|r - yc| below the cut-off, plus
-    // one row of margin); the row test is then a scalar compare
+    // one row of margin); the row test is then a scalar compare.  A strip
+    // whose columns all lie beyond the cut-off in x gets no source rows.
+    // The Gaussian is separable: per level each lane holds its columns'
+    // x factors gx = 2^(-ax ux^2) (and their sum over its owned columns), so
+    // a source row costs one exponential for the row's y factor.
     float xc[K], yc[K], cS[K];
     int srlo[K], srhi[K];
     const float wcut = sqrtf(a.cut / a.ay);
+    const float wcut_x = sqrtf(a.cut / a.ax);
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         const LaserTick t = plan[step0 + j];
@@ -183,6 +201,8 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         if (t.q_scale == 0.0) { srlo[j] = 1; srhi[j] = 0; }   // beam off: no source rows
     }
 
+    // per level and lane: the four values of the row holding the lane's peak
+    __shared__ float4 s_best[K][kWaves][kWave];
     LevelAcc acc[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j] = LevelAcc{kNegInf, 0x7fffffff, ny, -1, 0, 0.0, 0.f, 0.f};
@@ -198,6 +218,35 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         const int band = task / a.nstrips, strip = task % a.nstrips;
         const int L0 = strip * kOutCols - kOv;
         const int col4 = L0 + 4 * lane;
+        float gx[K][4], gxs[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            // strip columns [L0, L0 + kLoadCols) against |x - xc| <= wcut_x
+            // (one column of margin)
+            if (static_cast<float>(L0 + kLoadCols) < xc[j] - wcut_x - 1.f ||
+                static_cast<float>(L0) > xc[j] + wcut_x + 1.f) {
+                srlo[j] = 1;
+                srhi[j] = 0;
+            }
+            gxs[j] = 0.f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                // with the reflected halo, ghost columns take the source of
+                // the column they mirror
+                int cm = col4 + e;
+                if (kReflect) cm = cm < 0 ? -1 - cm : (cm >= nx ? 2 * nx - 1 - cm : cm);
+                const float ux = static_cast<float>(cm) - xc[j];
+                // v_exp_f32: results below 2^-126 flush to 0, far under
+                // FP32 resolution of any theta
+                gx[j][e] = __builtin_amdgcn_exp2f(-(a.ax * ux) * ux);
+                // the source sum covers the owned columns (lanes past column
+                // nx own nothing and are dropped later; only a straddling
+                // lane needs the per-column test)
+                const int cc = col4 + e;
+                if (BC == LPBF_BC_DIRICHLET) gxs[j] += (cc >= 1 && cc < nx - 1) ? gx[j][e] : 0.f;
+                else gxs[j] += (!RAGGED || cc < nx) ? gx[j][e] : 0.f;
+            }
+        }
         const bool ld_ok = col4 >= 0 && col4 < nx;
         const bool own_lane = lane >= 1 && lane <= kWave - 2 && col4 < nx;
         owner = own_lane;
@@ -266,17 +315,24 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             return (v.x + v.y) + (v.z + v.w);
         };
 
+        // reflected halo: lanes left of column 0 or right of column nx-1 load
+        // the nearest in-field quad (reversed when the row is consumed)
+        const int col4c = kReflect ? min(max(col4, 0), nx - 4) : min(max(col4, 0), a.ld - 4);
+        const bool rev = kReflect && (col4 < 0 || col4 >= nx);
         auto load_row = [&](int i) {
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (i < y_hi && ld_ok)
+            if (kReflect) {
+                if (i < y_hi)
+                    v = ld_row4((BUFS ? src : a.in) + static_cast<long long>(i) * a.ld + col4c);
+            } else if (i < y_hi && ld_ok) {
                 v = ld_row4((BUFS ? src : a.in) + static_cast<long long>(i) * a.ld + col4);
+            }
             return v;
         };
         // rows past y_hi are never consumed, so the row index is clamped
         // instead of tested; lanes outside [0, ld) load a clamped column (the
-        // values are never owned, and the edge fix-ups overwrite the ghost
-        // columns the owned cells read)
-        const int col4c = min(max(col4, 0), a.ld - 4);
+        // values are never owned, and the reflected halo or the edge fix-ups
+        // supply the ghost columns the owned cells read)
         auto load_row_clamped = [&](int i) {
             const int ii = min(i, y_hi - 1);
             return ld_row4((BUFS ? src : a.in) + static_cast<long long>(ii) * a.ld + col4c);
@@ -333,11 +389,7 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             const float4 c = win_mid(PHC, JP{});
             const float4 up = win_old(PHC, JP{});
             const float4 dn = win_new(PHC, JP{});
-            const float wl = from_left(c.w);    // column left of c.x
-            const float wr = from_right(c.x);   // column right of c.w
             const float cv[4] = {c.x, c.y, c.z, c.w};
-            const float xm[4] = {wl, c.x, c.y, c.z};
-            const float xp[4] = {c.y, c.z, c.w, wr};
             const float uv[4] = {up.x, up.y, up.z, up.w};
             const float dv[4] = {dn.x, dn.y, dn.z, dn.w};
             // theta' = P(theta) + S + cy*(up + down) + cx*(left + right) with
@@ -359,34 +411,27 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             // and only source rows touch the source sum
             const bool src_row = r >= srlo[j - 1] && r <= srhi[j - 1];
             if (src_row) {
-                float ssum = 0.f;
                 const float uy = static_cast<float>(r) - yc[j - 1];
-                const float ey = a.ay * uy * uy;
+                // the row's y factor times the source scale
+                const float gy = cS[j - 1] * __builtin_amdgcn_exp2f(-(a.ay * uy) * uy);
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const float ux = static_cast<float>(col4 + e) - xc[j - 1];
-                    // v_exp_f32: results below 2^-126 flush to 0, far under
-                    // FP32 resolution of any theta
-                    const float sv = cS[j - 1] * __builtin_amdgcn_exp2f(-fmaf(a.ax * ux, ux, ey));
-                    out4[e] += sv;
-                    // lanes past column nx own nothing and are dropped later;
-                    // only a straddling lane needs the per-column test
-                    const int cc = col4 + e;
-                    if (BC == LPBF_BC_DIRICHLET) ssum += (cc >= 1 && cc < nx - 1) ? sv : 0.f;
-                    else ssum += (!RAGGED || cc < nx) ? sv : 0.f;
-                }
+                for (int e = 0; e < 4; ++e) out4[e] = fmaf(gx[j - 1][e], gy, out4[e]);
                 // a select, not a nested branch (which would route the
                 // wave-uniform condition through a VGPR on every row)
-                acc[j - 1].SS += (own_row && !drow) ? ssum : 0.f;
+                acc[j - 1].SS = fmaf((own_row && !drow) ? gy : 0.f, gxs[j - 1], acc[j - 1].SS);
                 pin(acc[j - 1].SS);
             }
+            const float wl = from_left(c.w);    // column left of c.x
+            const float wr = from_right(c.x);   // column right of c.w
+            const float xm[4] = {wl, c.x, c.y, c.z};
+            const float xp[4] = {c.y, c.z, c.w, wr};
 #pragma unroll
             for (int e = 0; e < 4; ++e) out4[e] = fmaf(a.cx, xm[e] + xp[e], out4[e]);
             float4 res = make_float4(out4[0], out4[1], out4[2], out4[3]);
             const float4 P4 = make_float4(cen4[0], cen4[1], cen4[2], cen4[3]);
             if (drow) {
                 res = make_float4(a.theta_bc, a.theta_bc, a.theta_bc, a.theta_bc);
-            } else if (edge) {
+            } else if (!kReflect && edge) {
                 if (BC == LPBF_BC_DIRICHLET) fix_dirichlet(res);
                 else fix_mirror(res);
             }
@@ -400,12 +445,14 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                     for (int e = 0; e < 4; ++e)
                         if (e > laste) elem(mx, e) = kNegInf;
                 }
+                // running peak of the lane, branch-free: the value, the body
+                // row it came from, and that row's four values in LDS (the
+                // column is read back once, after the row loop)
                 const float m4 = max4(mx);
-                if (m4 > A.bv) {
-                    A.bv = m4;
-                    const int e = (mx.x == m4) ? 0 : (mx.y == m4) ? 1 : (mx.z == m4) ? 2 : 3;
-                    A.bi = r * nx + col4 + e;
-                }
+                const bool up = m4 > A.bv;
+                A.bv = up ? m4 : A.bv;
+                A.bi = up ? i : A.bi;
+                if (up) s_best[j - 1][wave][lane] = mx;
                 // molten cells and rows are counted for the wavefront from
                 // ballots, so they live in scalar registers
                 if (__builtin_amdgcn_ballot_w64(m4 > a.theta_m) & own_mask) {
@@ -446,7 +493,11 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             Z[(ph + 3) % 6] = GEN ? load_row(i + 3) : load_row_clamped(i + 3);
             float4& v = win_new(PHC, J0{});
             if (!GEN || i < y_hi) {
-                if (edge && BC == LPBF_BC_MIRROR) fix_mirror(v);
+                if (kReflect) {
+                    if (edge && rev) v = make_float4(v.w, v.z, v.y, v.x);
+                } else if (edge && BC == LPBF_BC_MIRROR) {
+                    fix_mirror(v);
+                }
                 if (i >= ya && i < yb) {
                     const bool drow = GEN && BC == LPBF_BC_DIRICHLET && (i == 0 || i == ny - 1);
                     S0 += drow ? 0.0 : static_cast<double>(row_sum(dir_cols(v)));
@@ -506,6 +557,18 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             __builtin_amdgcn_sched_barrier(0);
         }
         for (; i <= i_end; ++i) general(i);
+
+        // peak index: level j's best lane value came from row bi - j; the
+        // first of that row's four columns holding it
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            LevelAcc& A = acc[j];
+            if (A.bi != 0x7fffffff) {
+                const float4 v = s_best[j][wave][lane];
+                const int e = (v.x == A.bv) ? 0 : (v.y == A.bv) ? 1 : (v.z == A.bv) ? 2 : 3;
+                A.bi = (A.bi - (j + 1)) * nx + col4 + e;
+            }
+        }
     }
 
     // per level: wavefront then workgroup merge -> a.part[j*nwg + wg]
