@@ -165,15 +165,18 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
     const int task = wg * kWaves + wave;
     const int nx = a.nx, ny = a.ny;
     constexpr float kNegInf = -3.0e38f;
-    // Mirror edges on a width that is a multiple of 4 use a reflected halo:
-    // the ghost columns -4..-1 and nx..nx+3 are loaded as the mirror images
-    // of columns 3..0 and nx-1..nx-4 (one lane each, its four values
-    // reversed), and the ghost cells are updated like any other (the source
-    // at the mirrored column).  The update is symmetric, so every level's
-    // ghost column -1 equals column 0 bit for bit (and nx equals nx-1): the
-    // mirror condition holds without per-level fix-ups.  Each level spoils
-    // one more ghost column from the far side, so four stay exact for K <= 4.
-    constexpr bool kReflect = (BC == LPBF_BC_MIRROR) && !RAGGED;
+    // Mirror edges use a reflected halo: the ghost columns -4..-1 and
+    // nx..nx+3 enter as the mirror images of columns 3..0 and nx-1..nx-4,
+    // and the ghost cells are updated like any other (the source at the
+    // mirrored column).  The update is symmetric, so every level's ghost
+    // column -1 equals column 0 bit for bit (and nx equals nx-1): the mirror
+    // condition holds without per-level fix-ups.  Each level spoils one more
+    // ghost column from the far side, so four stay exact for K <= 4.  The
+    // left ghosts are one lane (its load reversed); on the right, a width
+    // that is a multiple of 4 puts them in one lane too, and otherwise they
+    // straddle two lanes and are filled once per loaded row from the lanes
+    // holding columns nx-1..nx-4 (fill_right).
+    constexpr bool kReflect = (BC == LPBF_BC_MIRROR);
 
     // per level: laser position in cell units, source scale, and the rows
     // [srlo, srhi] whose source can matter (|r - yc| below the cut-off, plus
@@ -234,7 +237,13 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                 // with the reflected halo, ghost columns take the source of
                 // the column they mirror
                 int cm = col4 + e;
-                if (kReflect) cm = cm < 0 ? -1 - cm : (cm >= nx ? 2 * nx - 1 - cm : cm);
+                if (kReflect) {
+                    // twice: on a width below 4 a mirrored column can itself
+                    // be a ghost
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        cm = cm < 0 ? -1 - cm : (cm >= nx ? 2 * nx - 1 - cm : cm);
+                }
                 const float ux = static_cast<float>(cm) - xc[j];
                 // v_exp_f32: results below 2^-126 flush to 0, far under
                 // FP32 resolution of any theta
@@ -289,6 +298,31 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
                 }
             }
         };
+        // Reflected halo on a ragged right edge: column nx sits at element
+        // pade (1..3) of lane padl, so the ghosts nx..nx+3 := nx-1..nx-4 lie
+        // in lanes padl and padl+1 and their sources in lanes padl-1 and
+        // padl (one or two lanes to the left).
+        auto fill_right = [&](float4& v) {
+            const bool L1 = lane == padl, L2 = lane == padl + 1;
+            if (pade == 1) {
+                // padl: x=nx-1 y=nx z=nx+1 w=nx+2; padl+1: x=nx+3
+                const float bw = from_left(v.w), bz = from_left(v.z);
+                const float cy = from_left(from_left(v.y));
+                if (L1) { v.y = v.x; v.z = bw; v.w = bz; }
+                if (L2) v.x = cy;
+            } else if (pade == 2) {
+                // padl: z=nx w=nx+1; padl+1: x=nx+2 y=nx+3
+                const float cw = from_left(from_left(v.w)), cz = from_left(from_left(v.z));
+                if (L1) { v.z = v.y; v.w = v.x; }
+                if (L2) { v.x = cw; v.y = cz; }
+            } else {
+                // padl: w=nx; padl+1: x=nx+1 y=nx+2 z=nx+3
+                const float by = from_left(v.y), bx = from_left(v.x);
+                const float cw = from_left(from_left(v.w));
+                if (L1) v.w = v.z;
+                if (L2) { v.x = by; v.y = bx; v.z = cw; }
+            }
+        };
         // Dirichlet: columns 0 and nx-1 := theta_bc (dir_cols keeps them out
         // of the sums).
         auto fix_dirichlet = [&](float4& v) {
@@ -317,7 +351,10 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
 
         // reflected halo: lanes left of column 0 or right of column nx-1 load
         // the nearest in-field quad (reversed when the row is consumed)
-        const int col4c = kReflect ? min(max(col4, 0), nx - 4) : min(max(col4, 0), a.ld - 4);
+        // nx4 - 4: the last quad that starts at or before column nx-1 (nx4 is
+        // nx rounded up to a multiple of 4, at most the pitch)
+        const int nx4 = (nx + 3) & ~3;
+        const int col4c = kReflect ? min(max(col4, 0), nx4 - 4) : min(max(col4, 0), a.ld - 4);
         const bool rev = kReflect && (col4 < 0 || col4 >= nx);
         auto load_row = [&](int i) {
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -494,7 +531,17 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
             float4& v = win_new(PHC, J0{});
             if (!GEN || i < y_hi) {
                 if (kReflect) {
-                    if (edge && rev) v = make_float4(v.w, v.z, v.y, v.x);
+                    if (edge) {
+                        if (rev) v = make_float4(v.w, v.z, v.y, v.x);
+                        if (RAGGED && has_right && pade != 0) fill_right(v);
+                        if (RAGGED && nx < 4) {
+                            // columns 0..3 hold right ghosts too: lane 0
+                            // mirrors lane 1's quad once that is filled
+                            const float4 r1 = make_float4(from_right(v.w), from_right(v.z),
+                                                          from_right(v.y), from_right(v.x));
+                            if (lane == 0) v = r1;
+                        }
+                    }
                 } else if (edge && BC == LPBF_BC_MIRROR) {
                     fix_mirror(v);
                 }

@@ -63,7 +63,8 @@ def _init(nx, ny, seed):
     return 300.0 + hot + rng.uniform(-0.5, 0.5, (ny, nx))
 
 
-@pytest.mark.parametrize("nx,ny", [(37, 5), (250, 40), (509, 133), (640, 64), (1000, 17)])
+@pytest.mark.parametrize("nx,ny", [(37, 5), (250, 40), (509, 133), (640, 64), (1000, 17),
+                                   (131, 21), (3, 9)])
 @pytest.mark.parametrize("bc", [0, 1])
 def test_depth_and_band_invariance(ora, nx, ny, bc):
     print(f"\n[case] f32 invariance nx={nx} ny={ny} bc={bc}", flush=True)
@@ -78,8 +79,11 @@ def test_depth_and_band_invariance(ora, nx, ny, bc):
         diff = np.argwhere(th != base)
         assert diff.size == 0, (depth, rows, len(diff), diff[:5].tolist())
         # FP32 per-lane partial sums group cells by tile, so the energy sums
-        # may differ by FP32 rounding of the partials (~1e-6 K per cell)
-        tol = 1e-5 * nx * ny
+        # may differ by FP32 rounding of the partials: ~1e-6 K per cell, and
+        # ~1e-6 of the summed theta where a few cells carry it (tiny plates;
+        # the loss sum is kc*sum(theta) - sum(P), so it inherits the rounding
+        # of sum(P), a sum of the size of sum(theta))
+        tol = 1e-5 * nx * ny + 1e-6 * float(np.abs(base).sum())
         for s_, (a_, b_) in enumerate(zip(sts, bst)):
             assert (a_.tmax, a_.arg, a_.ymin, a_.ymax, a_.molten) == \
                    (b_.tmax, b_.arg, b_.ymin, b_.ymax, b_.molten), (depth, rows, s_)
@@ -106,6 +110,31 @@ def test_error_against_fp64_oracle(ora, depth):
     assert err <= 0.05
     assert abs(sts[-1].tmax - rst[-1].tmax) <= 0.05
     assert abs(sts[-1].molten - rst[-1].molten) <= 3
+
+
+@pytest.mark.parametrize("nx", [297, 298, 299, 300])
+def test_mirror_edges_every_width_residue(ora, nx):
+    """Mirror edges on widths of each residue mod 4 (the right ghost columns
+    start at each of a lane's four elements) with the laser on the right
+    edge: FP32 at depth 4 within 0.05 K of the FP64 oracle."""
+    print(f"\n[case] f32 vs fp64 mirror edge nx={nx}", flush=True)
+    ny, n = 60, 200
+    s = _setup(ora, nx, ny)
+    s.proc.Tamb = s.proc.Tsub = 300.0
+    k = ora.make_const(s, ora.dt_cfl(s))
+    plan = ora.track_plan(s.proc.V, k.dt, s.dom.Lx, s.dom.Ly, n).copy()
+    plan[:, 0] = s.dom.Lx - 0.5 * k.dx       # beam centred on the last column
+    T0 = 300.0 + ora.noise_field(7, 0.0, 0.5, (ny, nx))
+    Tref, rst = ora.run_plan(k, plan, T0)
+    th, sts, _ = run_f32(k, plan, T0, n, 4)
+    err = np.abs(300.0 + th.astype(np.float64) - Tref).max()
+    print(f"max|dT| f32 vs f64 = {err:.3e} K, peak {Tref.max():.1f} K")
+    assert err <= 0.05
+    assert abs(sts[-1].tmax - rst[-1].tmax) <= 0.05
+    # the beam sits between two rows, so the peak is a near-tie of mirror
+    # cells: the reported cell must hold the peak to within the same bound
+    assert Tref.flat[sts[-1].arg] >= Tref.max() - 0.05
+    assert sts[-1].arg % nx >= nx - 2
 
 
 @pytest.mark.parametrize("depth,rows", [(3, 0), (4, 37)])
