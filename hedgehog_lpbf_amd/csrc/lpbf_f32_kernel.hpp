@@ -589,7 +589,15 @@ __device__ __forceinline__ void step_tile(const F32Args& a, const float* __restr
         // scheduling barriers keep the phases apart: interleaving them buys
         // no latency the two-plus resident waves do not already hide, and
         // costs registers
+        // quarter marks of this wavefront's rows, for its issue priority
+        const int span = i_end + 1 - y_lo;
+        const int q1 = y_lo + span / 4, q2 = y_lo + span / 2, q3 = y_lo + (3 * span) / 4;
         for (; i + 6 <= i_e; i += 6) {
+            // a wavefront that is ahead yields issue slots to the ones behind
+            if (i >= q3) __builtin_amdgcn_s_setprio(0);
+            else if (i >= q2) __builtin_amdgcn_s_setprio(1);
+            else if (i >= q1) __builtin_amdgcn_s_setprio(2);
+            else __builtin_amdgcn_s_setprio(3);
             body(std::integral_constant<int, 0>{}, Fast{}, i);
             __builtin_amdgcn_sched_barrier(0);
             body(std::integral_constant<int, 1>{}, Fast{}, i + 1);
