@@ -261,8 +261,16 @@ __global__ __launch_bounds__(kTbThreads) __attribute__((amdgpu_waves_per_eu(K <=
         };
 
         // six rows per trip, one per ring phase; rows past i_end fail every
-        // level's row test and load clamped rows
+        // level's row test and load clamped rows.  The wavefront lowers its
+        // issue priority at the quarter marks of its rows, so the waves that
+        // are behind take the issue slots and a SIMD's waves finish together
+        const int span = i_end + 1 - y_lo;
+        const int q1 = y_lo + span / 4, q2 = y_lo + span / 2, q3 = y_lo + (3 * span) / 4;
         for (int i = y_lo; i <= i_end; i += 6) {
+            if (i >= q3) __builtin_amdgcn_s_setprio(0);
+            else if (i >= q2) __builtin_amdgcn_s_setprio(1);
+            else if (i >= q1) __builtin_amdgcn_s_setprio(2);
+            else __builtin_amdgcn_s_setprio(3);
             body(std::integral_constant<int, 0>{}, i);
             body(std::integral_constant<int, 1>{}, i + 1);
             body(std::integral_constant<int, 2>{}, i + 2);
