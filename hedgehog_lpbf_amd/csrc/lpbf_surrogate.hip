@@ -317,10 +317,14 @@ __global__ __launch_bounds__(kT) void k_conv_mfma(const uint16_t* __restrict__ X
         // prologue, already waited for).  Operations issued after it: this
         // row's and the previous row's transfers (if any) and the three
         // rows' other operations; vector memory completes in issue order.
+        // A row past the window's last is written as zeros with no memory
+        // operation, so it does not count (counting it would let the wait
+        // pass before the transfer it guards has landed).
         if (row + 1 < ye && row - 2 >= ys) {
-            const bool dma_prev = row + 2 < ye;      // row - 1 issued one
-            if (dma && dma_prev) vm_wait<3 * kRowOps + 2 * kDmaOps>();
-            else if (dma_prev) vm_wait<3 * kRowOps + kDmaOps>();
+            const bool dma_mem = dma && row + 4 < kW;                 // this row's transfer
+            const bool dma_prev = row + 2 < ye && row + 3 < kW;       // row - 1's
+            if (dma_mem && dma_prev) vm_wait<3 * kRowOps + 2 * kDmaOps>();
+            else if (dma_mem || dma_prev) vm_wait<3 * kRowOps + kDmaOps>();
             else vm_wait<3 * kRowOps>();
         }
         // after the barrier every wavefront's part of input row + 2 is in

@@ -68,6 +68,29 @@ def test_forward_matches_reference():
     assert np.abs(got - ref).mean() <= 1e-3 * scale
 
 
+def test_forward_and_gradients_are_deterministic():
+    """Repeated passes over the same inputs give the same bits: every row a
+    conv workgroup reads from its LDS ring has landed before it is read
+    (including the rows past a window's last, which have no transfer)."""
+    print("\n[case] surrogate determinism, 8 windows x 4 passes", flush=True)
+    th = _params(3)
+    xs, tgt = _inputs(8, 4)
+    m = sg.Surrogate(params=th.astype(np.float32))
+    x0 = _x0(xs)
+    ys = [m.forward(x0).cpu().numpy() for _ in range(4)]
+    h.check()
+    for y in ys[1:]:
+        assert np.array_equal(y, ys[0])
+    t = torch.from_numpy(tgt.astype(np.float32)).cuda()
+    gs = []
+    for _ in range(3):
+        _, grad = m.loss_grad(x0, t)
+        gs.append(grad.cpu().numpy().copy())
+    h.check()
+    for g in gs[1:]:
+        assert np.array_equal(g, gs[0])
+
+
 def test_loss_and_gradients_match_reference():
     print("\n[case] surrogate loss + gradients, 1 window", flush=True)
     th = _params(3)
