@@ -288,9 +288,21 @@ __global__ __launch_bounds__(kCwThreads) void k_mr_coarse_window(F32Args a,
     // thread = column c of rows g, g + kCwGroups, ...
     const int c = threadIdx.x % kCwReg, g = threadIdx.x / kCwReg;
     const bool active = g < kCwGroups && c < RW;
-    if (active)
-        for (int r = g; r < RH; r += kCwGroups)
-            buf[0][r * kCwReg + c] = C[static_cast<size_t>(ry0 + r) * ldc + rx0 + c];
+    // every load of the region is issued before the first one is waited for
+    // (a loop storing each row to LDS as it arrives would pay the memory
+    // latency once per row)
+    constexpr int kCwRowsPer = (kCwReg + kCwGroups - 1) / kCwGroups;   // 6
+    float ld[kCwRowsPer];
+#pragma unroll
+    for (int q = 0; q < kCwRowsPer; ++q) {
+        const int r = g + q * kCwGroups;
+        ld[q] = (active && r < RH) ? C[static_cast<size_t>(ry0 + r) * ldc + rx0 + c] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < kCwRowsPer; ++q) {
+        const int r = g + q * kCwGroups;
+        if (active && r < RH) buf[0][r * kCwReg + c] = ld[q];
+    }
     __syncthreads();
     const float wcut = sqrtf(a.cut / a.ay);
     const int cl = max(c - 1, 0), cr = min(c + 1, RW - 1);
