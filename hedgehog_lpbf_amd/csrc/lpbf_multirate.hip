@@ -109,14 +109,11 @@ __global__ __launch_bounds__(kT) void k_mr_restrict(const float* __restrict__ fi
     }
 }
 
-__global__ __launch_bounds__(kT) void k_mr_gather(const lpbf_mr_window* __restrict__ win,
-                                                  uint16_t* __restrict__ x0,
-                                                  float* __restrict__ Dout,
-                                                  float* __restrict__ T0) {
-    const int w = blockIdx.y;
-    const int p = blockIdx.x * kT + threadIdx.x;
-    if (p >= kPix) return;
-    const lpbf_mr_window& W = win[w];
+// One pixel p of window w's network input; c1v is the coarse field after the
+// block, prolonged at the pixel's plate cell.
+__device__ inline void gather_pixel(const lpbf_mr_window& W, int w, int p, float c1v,
+                                    uint16_t* __restrict__ x0, float* __restrict__ Dout,
+                                    float* __restrict__ T0) {
     const int i = p / kW, j = p % kW;
     auto f = [&](int ii, int jj) {
         return W.fine[static_cast<size_t>(W.fy + ii) * W.ldf + W.fx + jj];
@@ -135,8 +132,6 @@ __global__ __launch_bounds__(kT) void k_mr_gather(const lpbf_mr_window* __restri
     const float detail = th - pr;
     // coarse increment, prolonged at plate cell (X, Y)
     const int X = W.ox + j, Y = W.oy + i;
-    const float c1v = W.ld1 > 0 ? prolong_block(W.c1, W.ld1, W.c1x, W.c1y, W.nxc, W.nyc, X, Y)
-                                : prolong(W.c1, W.ldc, W.nxc, W.nyc, X, Y);
     const float dC = c1v - prolong(W.c0, W.ldc, W.nxc, W.nyc, X, Y);
     // laser source map over the block: mean of q * exp(-2 r^2 / r0^2) as
     // exp2 of the cell offsets from the beam in cell units (FP32, far finer
@@ -162,6 +157,20 @@ __global__ __launch_bounds__(kT) void k_mr_gather(const lpbf_mr_window* __restri
     out[3] = make_uint4(0, 0, 0, 0);
     if (Dout) Dout[q] = dC;
     if (T0) T0[q] = th;
+}
+
+__global__ __launch_bounds__(kT) void k_mr_gather(const lpbf_mr_window* __restrict__ win,
+                                                  uint16_t* __restrict__ x0,
+                                                  float* __restrict__ Dout,
+                                                  float* __restrict__ T0) {
+    const int w = blockIdx.y;
+    const int p = blockIdx.x * kT + threadIdx.x;
+    if (p >= kPix) return;
+    const lpbf_mr_window& W = win[w];
+    const int X = W.ox + p % kW, Y = W.oy + p / kW;
+    const float c1v = W.ld1 > 0 ? prolong_block(W.c1, W.ld1, W.c1x, W.c1y, W.nxc, W.nyc, X, Y)
+                                : prolong(W.c1, W.ldc, W.nxc, W.nyc, X, Y);
+    gather_pixel(W, w, p, c1v, x0, Dout, T0);
 }
 
 __global__ __launch_bounds__(kT) void k_mr_target(const lpbf_mr_window* __restrict__ win,
@@ -272,17 +281,30 @@ __global__ __launch_bounds__(kT) void k_mr_fd_window(F32Args a, const float* __r
 // plate are not written (the prolongation never reads them).
 constexpr int kCwOut = kWc + 2;                 // 66
 constexpr int kCwReg = kCwOut + 2 * 4;          // 74
-constexpr int kCwThreads = 1024;
-constexpr int kCwGroups = kCwThreads / kCwReg;  // 13 row groups of 74 threads
+//
+// The step is fused into the network-input gather: workgroup g gathers the
+// window's fine rows 2g and 2g + 1 (the window origin is even), whose
+// prolongation reads rows g .. g + 2 of the 66 coarse rows above, so it steps
+// only those rows and 4 more on either side.  Every such row's four-step
+// dependency cone lies in the workgroup's region, and the region is clipped
+// at the whole region's edges as above, so the rows hold the same values as
+// one workgroup stepping the whole region.
+constexpr int kCwRows = 3;                      // coarse rows one workgroup's gather reads
+constexpr int kCwRegH = kCwRows + 2 * 4;        // 11
+constexpr int kCwGroups = kT / kCwReg;          // 3 row groups of 74 threads
+static_assert(kT == 2 * kW, "one fused-gather workgroup per two fine rows");
 template <bool RAD>
-__global__ __launch_bounds__(kCwThreads) void k_mr_coarse_window(F32Args a,
-                                                                 const float* __restrict__ C,
-                                                                 int ldc, int bx, int by,
-                                                                 const LaserTick* __restrict__ plan,
-                                                                 int step0, float* __restrict__ out) {
-    __shared__ float buf[2][kCwReg * kCwReg];
+__global__ __launch_bounds__(kT) void k_mr_gather_cw(F32Args a, const float* __restrict__ C,
+                                                     int ldc, int bx, int by,
+                                                     const LaserTick* __restrict__ plan,
+                                                     int step0,
+                                                     const lpbf_mr_window* __restrict__ win,
+                                                     uint16_t* __restrict__ x0,
+                                                     float* __restrict__ Dout) {
+    __shared__ float buf[2][kCwReg * kCwRegH];
     const int nx = a.nx, ny = a.ny;     // coarse plate
-    const int ry0 = max(by - 4, 0), ry1 = min(by + kCwOut + 4, ny);
+    const int g0 = blockIdx.x;          // coarse rows by + g0 .. by + g0 + 2
+    const int ry0 = max(by + g0 - 4, 0), ry1 = min(by + g0 + kCwRows + 4, ny);
     const int rx0 = max(bx - 4, 0), rx1 = min(bx + kCwOut + 4, nx);
     const int RH = ry1 - ry0, RW = rx1 - rx0;
     // thread = column c of rows g, g + kCwGroups, ...
@@ -291,7 +313,7 @@ __global__ __launch_bounds__(kCwThreads) void k_mr_coarse_window(F32Args a,
     // every load of the region is issued before the first one is waited for
     // (a loop storing each row to LDS as it arrives would pay the memory
     // latency once per row)
-    constexpr int kCwRowsPer = (kCwReg + kCwGroups - 1) / kCwGroups;   // 6
+    constexpr int kCwRowsPer = (kCwRegH + kCwGroups - 1) / kCwGroups;   // 4
     float ld[kCwRowsPer];
 #pragma unroll
     for (int q = 0; q < kCwRowsPer; ++q) {
@@ -340,13 +362,13 @@ __global__ __launch_bounds__(kCwThreads) void k_mr_coarse_window(F32Args a,
         __syncthreads();
         cur ^= 1;
     }
+    // the gather of fine rows 2 g0 and 2 g0 + 1, prolonging from the region
     const float* A = buf[cur];
-    for (int q = threadIdx.x; q < kCwOut * kCwOut; q += kCwThreads) {
-        const int v = q / kCwOut, u = q - v * kCwOut;
-        const int Y = by + v, X = bx + u;
-        if (Y >= 0 && Y < ny && X >= 0 && X < nx)
-            out[q] = A[(Y - ry0) * kCwReg + X - rx0];
-    }
+    const lpbf_mr_window& W = win[0];
+    const int p = g0 * kT + threadIdx.x;
+    const int X = W.ox + p % kW, Y = W.oy + p / kW;
+    const float c1v = prolong_block(A, kCwReg, rx0, ry0, W.nxc, W.nyc, X, Y);
+    gather_pixel(W, 0, p, c1v, x0, Dout, nullptr);
 }
 
 // Window after the block from the network: base (window + prolonged coarse
@@ -511,7 +533,6 @@ struct lpbf_mr_run {
     float* Wb[2] = {nullptr, nullptr};
     float* Wt = nullptr;
     float* Wf = nullptr;                 // the FD window after a block
-    float* Cw = nullptr;                 // coarse cells around the window after a block
     float* D = nullptr;
     float* y = nullptr;
     void* x0 = nullptr;
@@ -535,7 +556,6 @@ struct lpbf_mr_run {
                         static_cast<void*>(d_plan), static_cast<void*>(d_cplan),
                         static_cast<void*>(d_win)})
             if (p) (void)hipFree(p);
-        if (Cw) (void)hipFree(Cw);
         if (md_host) (void)hipHostFree(md_host);
         if (ev_a) (void)hipEventDestroy(ev_a);
         if (ev_b) (void)hipEventDestroy(ev_b);
@@ -578,14 +598,19 @@ struct lpbf_mr_run {
         }
     }
 
-    void coarse_window(int b) {
+    // The network input of block b (and its coarse increment D), with the
+    // window-local coarse step computed inside the gather.
+    void gather_window(int b) {
         const int bx = ox[b] / 2 - 1, by = oy[b] / 2 - 1;
         if (ac.a4 != 0.f)
-            hipLaunchKernelGGL(k_mr_coarse_window<true>, dim3(1), dim3(kCwThreads), 0, s, ac,
-                               C[cc], ldc, bx, by, d_cplan, 4 * b, Cw);
+            hipLaunchKernelGGL(k_mr_gather_cw<true>, dim3(kPix / kT), dim3(kT), 0, s, ac, C[cc],
+                               ldc, bx, by, d_cplan, 4 * b, d_win + b,
+                               static_cast<uint16_t*>(x0), D);
         else
-            hipLaunchKernelGGL(k_mr_coarse_window<false>, dim3(1), dim3(kCwThreads), 0, s, ac,
-                               C[cc], ldc, bx, by, d_cplan, 4 * b, Cw);
+            hipLaunchKernelGGL(k_mr_gather_cw<false>, dim3(kPix / kT), dim3(kT), 0, s, ac, C[cc],
+                               ldc, bx, by, d_cplan, 4 * b, d_win + b,
+                               static_cast<uint16_t*>(x0), D);
+        check_launch();
     }
 
     // One block.  The coarse plate's four steps run on s2 while the window
@@ -607,8 +632,7 @@ struct lpbf_mr_run {
         const float* src = nullptr;
         bool keep_fd = !net;
         if (net) {
-            coarse_window(b);
-            lpbf_mr_gather(d_win + b, 1, x0, D, nullptr, s);
+            gather_window(b);
             lpbf_sg_forward(pack, x0, 1, sg_work, y, s);
             ++cnt.calls;
             src = nullptr;   // formed on the fly by the post kernel
@@ -875,7 +899,6 @@ int lpbf_mr_create(const lpbf_mr_config* cfg, const LaserTick* h_plan, const voi
         if (r->depth_c < 4) r->C2 = r->alloc<float>(cbytes);
         r->Wt = r->alloc<float>(kPix);
         r->Wf = r->alloc<float>(kPix);
-        r->Cw = r->alloc<float>(kCwOut * kCwOut);
         r->D = r->alloc<float>(kPix);
         r->y = r->alloc<float>(kPix);
         r->x0 = r->alloc<uint16_t>(static_cast<size_t>(kPix) * kCh);
@@ -897,10 +920,7 @@ int lpbf_mr_create(const lpbf_mr_config* cfg, const LaserTick* h_plan, const voi
             w = lpbf_mr_window{};
             w.fine = r->Wb[b & 1];
             w.c0 = r->C[b & 1];
-            w.c1 = r->Cw;   // the window-local coarse step's result
-            w.ld1 = kCwOut;
-            w.c1x = r->ox[b] / 2 - 1;
-            w.c1y = r->oy[b] / 2 - 1;
+            // c1 stays empty: the run's gather steps the coarse cells it reads
             w.plan = r->d_plan;
             w.dx = k.dx;
             w.dy = k.dy;
